@@ -1,0 +1,192 @@
+"""Benchmark of the MI355X LULESH step.
+
+A step is one LULESH cycle (TimeIncrement + LagrangeLeapFrog) of the
+benchmark workload: the Sedov blast problem at 150^3 elements (3.375 M
+zones, 151^3 nodes) with the LULESH 2.0 multi-region layout (11 regions,
+balance 1, cost 1, seeded region assignment) and LULESH 2.0 initial
+conditions, in float64.  The domain is built once and stays resident on the
+GPU; the timed region is K consecutive cycles after W warm-up cycles.
+
+Prints one JSON line: zones*cycles/s of the timed cycles, the roofline of
+the dominant kernel (algorithmic bytes per launch over its mean duration,
+from HIP events on the domain's stream) and the reference LULESH 2.0.3
+(oracle/_ref/lulesh2, built from the reference sources) timed on this
+host's cores on a 20-cycle sample of the same workload.
+
+    python bench.py [--steps K] [--warmup W] [--nx N] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "lulesh-accelerate_amd"))
+
+METRIC = ("zones·cycles/s, plain FP64 vs batched/mixed/learned-Q at "
+          "stated error; HBM %")
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per element and per node of each kernel: the per-phase
+# figures of the reference path's byte model (each array a phase reads or
+# writes counted once, float64 = 8 B, int32 = 4 B; 2.0 adds the 4-byte
+# region index to the Q-region, EOS and dt phases), assigned to the kernel
+# that performs the phase.  Sum: 520 B/element + 296 B/node per cycle.
+KERNEL_BYTES = {
+    # stress + hourglass: p,q,v,volo,ss,elemMass + nodelist; reads x..zd
+    "k_corner_forces": (80, 48),
+    # force write fx,fy,fz (24) + acceleration/BC/velocity/position (128)
+    "k_nodal_update": (0, 152),
+    # kinematics (80 + 48 node) + monotonic-Q gradients (96 + 48 node)
+    "k_kinematics_q": (176, 96),
+    # Q region 124+4, EOS 104+4, dt constraints 24+4
+    "k_q_eos_dt": (264, 0),
+    "k_time_increment": (0, 0),
+    "k_dt_finish": (0, 0),
+}
+
+
+def host_cores() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(nx: int, cycles: int) -> dict | None:
+    """The reference LULESH 2.0.3 on this host's cores, same workload,
+    first `cycles` cycles; the reference times its own cycle loop."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "lulesh2")
+    cores = host_cores()
+    if os.path.exists(exe):
+        env = dict(os.environ, OMP_NUM_THREADS=str(cores),
+                   OMP_PROC_BIND="close", OMP_PLACES="cores")
+        out = subprocess.run([exe, "-s", str(nx), "-i", str(cycles)],
+                             capture_output=True, text=True, env=env,
+                             timeout=600)
+        m = re.search(r"Grind time \(us/z/c\)\s*=\s*([0-9.eE+-]+)", out.stdout)
+        c = re.search(r"Iteration count\s*=\s*(\d+)", out.stdout)
+        if out.returncode == 0 and m and c:
+            grind_us = float(m.group(1))
+            return {"value": 1.0e6 / grind_us, "unit": "zones·cycles/s",
+                    "cores": cores, "kind": "reference",
+                    "sample": (f"lulesh2 (reference/C/lulesh-2.0.3, g++ -O3 "
+                               f"-fopenmp) -s {nx} -i {c.group(1)}, "
+                               f"OMP_NUM_THREADS={cores}; reference's own "
+                               "loop timer")}
+    # the oracle restatement when the reference binary is unavailable
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import lulesh_oracle
+    o = lulesh_oracle.Oracle(nx, mode=2, threads=cores)
+    t0 = time.perf_counter()
+    o.run(cycles)
+    dt = time.perf_counter() - t0
+    return {"value": nx ** 3 * o.cycle / dt, "unit": "zones·cycles/s",
+            "cores": cores, "kind": "port",
+            "sample": f"oracle restatement, {nx}^3, {o.cycle} cycles, "
+                      f"{cores} OpenMP threads"}
+
+
+def load_traffic(kernel: str, nx: int) -> float | None:
+    """HBM bytes per launch of `kernel` measured by rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), committed under
+    profiles/ by the profiling runs; None when not measured."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        data = json.load(open(path))
+        return float(data[f"{kernel}@{nx}"]["bytes_per_launch"])
+    except (KeyError, ValueError, TypeError):
+        return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=150)
+    ap.add_argument("--cpu-cycles", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not "
+              "supported (the LULESH step here is a single-GPU build)",
+              file=sys.stderr)
+        return 2
+
+    import lulesh_amd
+
+    nx, K, W = args.nx, args.steps, args.warmup
+    d = lulesh_amd.Domain(nx, ic_mode=2, num_reg=11, balance=1, cost=1,
+                          max_cycles=W + K)
+    if W:
+        d.step(W)
+    d.set_kernel_timing(True)
+    t0 = time.perf_counter()
+    rc, done = d.step(K)  # synchronous: device idle before and after
+    wall = time.perf_counter() - t0
+    if rc != 0 or done != K:
+        print(f"bench: step returned rc={rc}, {done} of {K} cycles",
+              file=sys.stderr)
+        return 1
+    times = d.kernel_times()
+    st = d.stats()
+    d.close()
+
+    ne, nn = nx ** 3, (nx + 1) ** 3
+    zones_cycles = ne * K
+    value = zones_cycles / wall
+    # dominant kernel and its roofline
+    per_kernel = {k: (ms / n if n else 0.0) for k, (ms, n) in times.items()}
+    dom = max(per_kernel, key=lambda k: times[k][0])
+    be, bn = KERNEL_BYTES[dom]
+    alg_bytes = be * ne + bn * nn
+    achieved = alg_bytes / (per_kernel[dom] * 1e-3) / 1e9
+    b_cycle = 520 * ne + 296 * nn
+    roof_value = HBM_PEAK_GBS * 1e9 / b_cycle * ne
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "zones·cycles/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": wall * 1e3 / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: deterministic Sedov blast initial conditions",
+        "config": {"workload": f"sedov_{nx}^3_lulesh2.0_regions11_balance1_cost1",
+                   "nx": nx, "zones": ne, "nodes": nn, "ic": "LULESH 2.0.3",
+                   "num_reg": 11, "balance": 1, "cost": 1,
+                   "cycles_timed": K, "first_timed_cycle": W + 1,
+                   "final_cycle": st.cycle},
+        "roofline": {"bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": load_traffic(dom, nx),
+                     "kernel": dom, "alg_bytes_per_launch": alg_bytes,
+                     "mean_ms": per_kernel[dom]},
+        "cycle_roofline": {"b_alg_per_cycle": b_cycle,
+                           "roofline_zones_cycles_per_s": roof_value,
+                           "frac": value / roof_value},
+        "kernels_mean_ms": {k: round(v, 6) for k, v in per_kernel.items()},
+        "gpu_event_ms_per_step": sum(ms for ms, _ in times.values()) / K,
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(nx, args.cpu_cycles)
+    print(json.dumps(result))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,158 @@
+/*
+ * lulesh_mi355x.h -- C ABI of the MI355X LULESH Lagrange step.
+ *
+ * One call of lulesh_step() advances the Sedov blast problem by whole
+ * cycles, each cycle being TimeIncrement followed by LagrangeLeapFrog, the
+ * loop body of the reference drivers:
+ *   reference/C/LULESH-OMP.cc:3179-3188           (LULESH 1.0)
+ *   reference/C/lulesh-2.0.3/lulesh.cc:2771-2781  (LULESH 2.0.3)
+ *   src/Main.hs:90-109, src/LULESH.hs:37-53       (the Accelerate step)
+ * Every phase runs as a HIP kernel on one gfx950 GPU; the domain state stays
+ * resident in device memory between calls.
+ *
+ * Plain C types only: the Haskell driver binds these with `foreign import
+ * ccall` (the pattern of src/VisIt.hs:71-86) and the C++ harness links them
+ * directly.  All calls are synchronous on return and belong to the calling
+ * host thread; one domain must not be used by two threads at once.
+ *
+ * Return codes mirror the reference's exit codes
+ * (reference/C/LULESH-OMP.cc:82, lulesh-2.0.3/lulesh.h:41).
+ */
+#ifndef LULESH_MI355X_H
+#define LULESH_MI355X_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LULESH_OK 0
+#define LULESH_VOLUME_ERROR (-1) /* VolumeError: a volume went <= 0 */
+#define LULESH_QSTOP_ERROR (-2)  /* QStopError: q exceeded qstop */
+#define LULESH_BAD_ARG (-3)
+#define LULESH_HIP_ERROR (-4)
+
+/* Initial conditions.  1.0: e(0) = 3.948746e7, dt0 = 1e-7, one region
+ * (reference/C/LULESH-OMP.cc:3052-3112).  2.0: e(0) scaled by (nx/45)^3,
+ * dt0 from the CFL estimate, num_reg seeded regions with cost-dependent EOS
+ * repetitions (reference/C/lulesh-2.0.3/lulesh-init.cc:16-184, 377-484;
+ * lulesh.cc:2357-2430). */
+#define LULESH_IC_1_0 1
+#define LULESH_IC_2_0 2
+
+typedef struct lulesh_domain lulesh_domain;
+
+/* The run options of lulesh-2.0.3's command line
+ * (reference/C/lulesh-2.0.3/lulesh-util.cc:60-168; lulesh.cc:2703-2711). */
+typedef struct {
+    int nx;          /* -s: elements along each edge of the cube (>= 1) */
+    int ic_mode;     /* LULESH_IC_1_0 or LULESH_IC_2_0 */
+    int num_reg;     /* -r: regions (2.0 only), default 11 */
+    int balance;     /* -b: region size balance (2.0 only), default 1 */
+    int cost;        /* -c: extra cost of the costly regions, default 1 */
+    int max_cycles;  /* -i: stop after this many cycles, default 9999999 */
+    double stoptime; /* stop when time >= stoptime, default 1e-2 */
+} lulesh_config;
+
+/* Scalars of the current state. */
+typedef struct {
+    double time;       /* domain.time() */
+    double deltatime;  /* domain.deltatime() */
+    double dtcourant;  /* domain.dtcourant() */
+    double dthydro;    /* domain.dthydro() */
+    double e_origin;   /* e(0), the "Final Origin Energy" */
+    int cycle;         /* domain.cycle() */
+    int done;          /* time >= stoptime or cycle >= max_cycles */
+    int num_elem;
+    int num_node;
+} lulesh_stats;
+
+/* Field identifiers for lulesh_get_field / lulesh_set_field.  Node fields
+ * have num_node values, element fields num_elem; both in the reference's
+ * order (x fastest, then y, then z). */
+enum {
+    LULESH_F_X = 0, LULESH_F_Y, LULESH_F_Z,
+    LULESH_F_XD, LULESH_F_YD, LULESH_F_ZD,
+    LULESH_F_NODAL_MASS,
+    LULESH_F_E = 16, LULESH_F_P, LULESH_F_Q, LULESH_F_QL, LULESH_F_QQ,
+    LULESH_F_V, LULESH_F_VOLO, LULESH_F_DELV, LULESH_F_VDOV, LULESH_F_AREALG,
+    LULESH_F_SS, LULESH_F_ELEM_MASS, LULESH_F_VNEW,
+    LULESH_F_DELV_XI, LULESH_F_DELV_ETA, LULESH_F_DELV_ZETA,
+    LULESH_F_DELX_XI, LULESH_F_DELX_ETA, LULESH_F_DELX_ZETA
+};
+
+/* Fills the reference defaults: nx 30, 2.0 ICs, 11 regions, balance 1,
+ * cost 1, no cycle limit, stoptime 1e-2. */
+void lulesh_config_default(lulesh_config *cfg);
+
+/* Builds the mesh, boundary conditions, regions and initial conditions of
+ * Domain::Domain (reference/C/lulesh-2.0.3/lulesh-init.cc:16-184;
+ * reference/C/LULESH-OMP.cc:2960-3171) and uploads them to the GPU. */
+int lulesh_create(const lulesh_config *cfg, lulesh_domain **out);
+void lulesh_destroy(lulesh_domain *d);
+
+/* Advances up to ncycles cycles, stopping early at stoptime/max_cycles or on
+ * an error; *cycles_done (may be NULL) receives the cycles completed.
+ * Replaces one or more turns of the reference loop
+ * (reference/C/lulesh-2.0.3/lulesh.cc:2771-2781) and of `simulate`
+ * (src/Main.hs:90-109). */
+int lulesh_step(lulesh_domain *d, int ncycles, int *cycles_done);
+
+/* Runs to completion (time >= stoptime or cycle >= max_cycles). */
+int lulesh_run(lulesh_domain *d, int *cycles_done);
+
+/* State in the Accelerate step's terms (src/Domain.hs:11-21): the domain
+ * tuple (x, xd, e, p, q, v, ss, t, dt).  Host arrays, copied in/out. */
+int lulesh_upload_state(lulesh_domain *d, const double *x, const double *y,
+                        const double *z, const double *xd, const double *yd,
+                        const double *zd, const double *e, const double *p,
+                        const double *q, const double *v, const double *ss,
+                        double time, double deltatime);
+int lulesh_download_state(lulesh_domain *d, double *x, double *y, double *z,
+                          double *xd, double *yd, double *zd, double *e,
+                          double *p, double *q, double *v, double *ss,
+                          double *time, double *deltatime);
+
+/* One field, copied to/from a host buffer of num_node or num_elem values. */
+int lulesh_get_field(lulesh_domain *d, int field, double *host_out);
+int lulesh_set_field(lulesh_domain *d, int field, const double *host_in);
+
+/* Scalars; lulesh_set_scalars restores a checkpoint's time, dt, cycle and
+ * constraints (any NULL pointer leaves that value unchanged). */
+int lulesh_get_stats(lulesh_domain *d, lulesh_stats *out);
+int lulesh_set_scalars(lulesh_domain *d, const double *time,
+                       const double *deltatime, const double *dtcourant,
+                       const double *dthydro, const int *cycle);
+
+/* Region of each element (1-based, the reference's regNumList). */
+int lulesh_get_regions(lulesh_domain *d, int *reg_num_list);
+
+/* Plane-0 energy symmetry of the reference's final check
+ * (reference/C/lulesh-2.0.3/lulesh-util.cc:189-208). */
+int lulesh_symmetry(lulesh_domain *d, double *max_abs_diff,
+                    double *total_abs_diff, double *max_rel_diff);
+
+/* Per-kernel timing with HIP events on the domain's stream.  While enabled,
+ * every launch of lulesh_step is bracketed by events; lulesh_kernel_times
+ * returns, per kernel, the accumulated milliseconds and launch count. */
+int lulesh_kernel_count(void);
+const char *lulesh_kernel_name(int k);
+int lulesh_set_kernel_timing(lulesh_domain *d, int enable);
+int lulesh_kernel_times(lulesh_domain *d, double *ms, long long *launches);
+
+/* Milliseconds between the HIP events bracketing the last lulesh_step call
+ * (0 when timing was never enabled). */
+double lulesh_last_step_ms(lulesh_domain *d);
+
+/* Message of the last error on this domain ("" when none). */
+const char *lulesh_last_error(const lulesh_domain *d);
+
+/* The library's cube root (lulesh_cbrt.h) evaluated on the GPU, and IEEE
+ * sqrt / division, for n inputs: a math-parity probe for the tests. */
+int lulesh_device_math(int op, int n, const double *a, const double *b,
+                       double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LULESH_MI355X_H */

@@ -1,0 +1,74 @@
+"""Builds the in-tree native artefacts of the package.
+
+  liblulesh_mi355x.so   HIP kernels + C ABI (include/lulesh_mi355x.h), gfx950
+  lulesh_mi355x         C++ command-line harness linked against the library
+
+Both land next to this file so they travel with the source tree.  Device
+code is compiled without FMA contraction (the kernels reproduce the
+reference's rounding) and without packed-FP32 instructions.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+LIB = os.path.join(PKG, "liblulesh_mi355x.so")
+HARNESS = os.path.join(PKG, "lulesh_mi355x")
+
+HIP_SOURCES = ["lulesh_kernels.hip", "lulesh_abi.hip", "lulesh_mesh.cpp"]
+HEADERS = ["lulesh_kernels.hpp", "lulesh_elem.hpp", "lulesh_cbrt.h",
+           "lulesh_mesh.hpp"]
+
+COMMON_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I" + INCLUDE, "-I" + CSRC,
+]
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc",
+                 shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd: list[str]) -> None:
+    out = subprocess.run(cmd, capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("build failed: " + " ".join(cmd) + "\n" +
+                           out.stdout + out.stderr)
+
+
+def build_library() -> str:
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    tmp = LIB + ".tmp"
+    _run([hipcc()] + COMMON_FLAGS + ["-fPIC", "-shared", "-o", tmp] + srcs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_harness() -> str:
+    src = os.path.join(PKG, "harness", "lulesh_main.cc")
+    tmp = HARNESS + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + INCLUDE, "-o", tmp, src,
+          "-L" + PKG, "-llulesh_mi355x", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, HARNESS)
+    return HARNESS
+
+
+def build_all() -> list[str]:
+    return [build_library(), build_harness()]
+
+
+if __name__ == "__main__":
+    for p in build_all():
+        print("built", p)
+    sys.exit(0)

@@ -1,0 +1,734 @@
+// lulesh_abi.hip -- the C ABI of include/lulesh_mi355x.h.
+//
+// Owns the device state of one domain, builds it from the host mesh, queues
+// cycles of kernels on the domain's stream and copies fields in and out.
+#include "lulesh_mi355x.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "lulesh_kernels.hpp"
+#include "lulesh_mesh.hpp"
+
+using namespace lulesh;
+
+namespace {
+
+enum KernelId {
+    KID_TIME_INCREMENT = 0,
+    KID_CORNER_FORCES,
+    KID_NODAL_UPDATE,
+    KID_KINEMATICS_Q,
+    KID_Q_EOS_DT,
+    KID_DT_FINISH,
+    KID_COUNT
+};
+
+const char *const kKernelNames[KID_COUNT] = {
+    "k_time_increment", "k_corner_forces", "k_nodal_update",
+    "k_kinematics_q",   "k_q_eos_dt",      "k_dt_finish",
+};
+
+constexpr int kBlock = 256;
+constexpr int kBatch = 32;  // cycles queued per host round trip in run()
+
+size_t round_up(size_t n, size_t m) { return (n + m - 1) / m * m; }
+
+struct TimedLaunch {
+    int kernel;
+    hipEvent_t start, stop;
+};
+
+}  // namespace
+
+struct lulesh_domain {
+    lulesh_config cfg{};
+    int nx = 0;
+    int64_t num_elem = 0, num_node = 0;
+    int num_reg = 1;
+    std::vector<int32_t> reg_num_list;
+
+    DevFields f{};
+    DevScalars *s = nullptr;
+    PhysParams prm{};
+    void *arena = nullptr;
+    hipStream_t stream = nullptr;
+
+    bool timing = false;
+    std::vector<hipEvent_t> event_pool;
+    size_t events_used = 0;
+    std::vector<TimedLaunch> pending;
+    double kernel_ms[KID_COUNT] = {};
+    long long kernel_launches[KID_COUNT] = {};
+    hipEvent_t step_begin = nullptr, step_end = nullptr;
+    double last_step_ms = 0.0;
+
+    std::string err;
+};
+
+namespace {
+
+int fail(lulesh_domain *d, int code, const std::string &msg)
+{
+    if (d) {
+        d->err = msg;
+    }
+    return code;
+}
+
+#define HIP_TRY(d, expr)                                                     \
+    do {                                                                     \
+        hipError_t e_ = (expr);                                              \
+        if (e_ != hipSuccess) {                                              \
+            return fail((d), LULESH_HIP_ERROR,                               \
+                        std::string(#expr ": ") + hipGetErrorString(e_));    \
+        }                                                                    \
+    } while (0)
+
+hipEvent_t next_event(lulesh_domain *d)
+{
+    if (d->events_used == d->event_pool.size()) {
+        hipEvent_t ev = nullptr;
+        if (hipEventCreate(&ev) != hipSuccess) {
+            return nullptr;
+        }
+        d->event_pool.push_back(ev);
+    }
+    return d->event_pool[d->events_used++];
+}
+
+// launches one kernel, bracketed by events when kernel timing is on
+template <typename F>
+int launch(lulesh_domain *d, int kid, F &&body)
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    if (d->timing) {
+        a = next_event(d);
+        b = next_event(d);
+        if (!a || !b) {
+            return fail(d, LULESH_HIP_ERROR, "hipEventCreate failed");
+        }
+        HIP_TRY(d, hipEventRecord(a, d->stream));
+    }
+    body();
+    HIP_TRY(d, hipGetLastError());
+    if (d->timing) {
+        HIP_TRY(d, hipEventRecord(b, d->stream));
+        d->pending.push_back({kid, a, b});
+    }
+    return LULESH_OK;
+}
+
+int queue_cycle(lulesh_domain *d)
+{
+    const unsigned ge = unsigned((d->num_elem + kBlock - 1) / kBlock);
+    const unsigned gn = unsigned((d->num_node + kBlock - 1) / kBlock);
+    hipStream_t st = d->stream;
+    DevScalars *s = d->s;
+    const DevFields &f = d->f;
+    const PhysParams &p = d->prm;
+    int rc;
+    if ((rc = launch(d, KID_TIME_INCREMENT, [&] {
+             hipLaunchKernelGGL(k_time_increment, dim3(1), dim3(64), 0, st, s);
+         })))
+        return rc;
+    if ((rc = launch(d, KID_CORNER_FORCES, [&] {
+             hipLaunchKernelGGL(k_corner_forces, dim3(ge), dim3(kBlock), 0, st, f, s, p);
+         })))
+        return rc;
+    if ((rc = launch(d, KID_NODAL_UPDATE, [&] {
+             hipLaunchKernelGGL(k_nodal_update, dim3(gn), dim3(kBlock), 0, st, f, s, p);
+         })))
+        return rc;
+    if ((rc = launch(d, KID_KINEMATICS_Q, [&] {
+             hipLaunchKernelGGL(k_kinematics_q, dim3(ge), dim3(kBlock), 0, st, f, s, p);
+         })))
+        return rc;
+    if ((rc = launch(d, KID_Q_EOS_DT, [&] {
+             hipLaunchKernelGGL(k_q_eos_dt, dim3(ge), dim3(kBlock), 0, st, f, s, p);
+         })))
+        return rc;
+    if ((rc = launch(d, KID_DT_FINISH, [&] {
+             hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(64), 0, st, s, p.ic_mode);
+         })))
+        return rc;
+    return LULESH_OK;
+}
+
+int read_scalars(lulesh_domain *d, DevScalars *out)
+{
+    HIP_TRY(d, hipMemcpyAsync(out, d->s, sizeof(DevScalars), hipMemcpyDeviceToHost,
+                              d->stream));
+    HIP_TRY(d, hipStreamSynchronize(d->stream));
+    return LULESH_OK;
+}
+
+int write_scalars(lulesh_domain *d, const DevScalars &in)
+{
+    HIP_TRY(d, hipMemcpyAsync(d->s, &in, sizeof(DevScalars), hipMemcpyHostToDevice,
+                              d->stream));
+    HIP_TRY(d, hipStreamSynchronize(d->stream));
+    return LULESH_OK;
+}
+
+// folds the event pairs of finished launches into the per-kernel totals
+int collect_timings(lulesh_domain *d)
+{
+    for (const TimedLaunch &t : d->pending) {
+        float ms = 0.f;
+        HIP_TRY(d, hipEventElapsedTime(&ms, t.start, t.stop));
+        d->kernel_ms[t.kernel] += ms;
+        d->kernel_launches[t.kernel] += 1;
+    }
+    d->pending.clear();
+    d->events_used = 0;
+    return LULESH_OK;
+}
+
+struct FieldRef {
+    double *ptr;
+    int64_t n;
+};
+
+FieldRef field_ref(lulesh_domain *d, int id)
+{
+    const DevFields &f = d->f;
+    const int64_t ne = d->num_elem, nn = d->num_node;
+    switch (id) {
+    case LULESH_F_X: return {f.x, nn};
+    case LULESH_F_Y: return {f.y, nn};
+    case LULESH_F_Z: return {f.z, nn};
+    case LULESH_F_XD: return {f.xd, nn};
+    case LULESH_F_YD: return {f.yd, nn};
+    case LULESH_F_ZD: return {f.zd, nn};
+    case LULESH_F_NODAL_MASS: return {f.nodal_mass, nn};
+    case LULESH_F_E: return {f.e, ne};
+    case LULESH_F_P: return {f.p, ne};
+    case LULESH_F_Q: return {f.q, ne};
+    case LULESH_F_QL: return {f.ql, ne};
+    case LULESH_F_QQ: return {f.qq, ne};
+    case LULESH_F_V: return {f.v, ne};
+    case LULESH_F_VOLO: return {f.volo, ne};
+    case LULESH_F_DELV: return {f.delv, ne};
+    case LULESH_F_VDOV: return {f.vdov, ne};
+    case LULESH_F_AREALG: return {f.arealg, ne};
+    case LULESH_F_SS: return {f.ss, ne};
+    case LULESH_F_ELEM_MASS: return {f.elem_mass, ne};
+    case LULESH_F_VNEW: return {f.vnew, ne};
+    case LULESH_F_DELV_XI: return {f.delv_xi, ne};
+    case LULESH_F_DELV_ETA: return {f.delv_eta, ne};
+    case LULESH_F_DELV_ZETA: return {f.delv_zeta, ne};
+    case LULESH_F_DELX_XI: return {f.delx_xi, ne};
+    case LULESH_F_DELX_ETA: return {f.delx_eta, ne};
+    case LULESH_F_DELX_ZETA: return {f.delx_zeta, ne};
+    default: return {nullptr, 0};
+    }
+}
+
+int copy_in(lulesh_domain *d, double *dst, const double *src, int64_t n)
+{
+    HIP_TRY(d, hipMemcpyAsync(dst, src, size_t(n) * sizeof(double),
+                              hipMemcpyHostToDevice, d->stream));
+    return LULESH_OK;
+}
+
+int copy_out(lulesh_domain *d, double *dst, const double *src, int64_t n)
+{
+    HIP_TRY(d, hipMemcpyAsync(dst, src, size_t(n) * sizeof(double),
+                              hipMemcpyDeviceToHost, d->stream));
+    return LULESH_OK;
+}
+
+int build_device_domain(lulesh_domain *d, const HostMesh &m)
+{
+    const size_t ne = size_t(m.num_elem), nn = size_t(m.num_node);
+    const size_t pe = round_up(ne, 64), pn = round_up(nn, 64);
+    // arena: 7 node arrays, 19 element arrays, 48 corner arrays, scalars,
+    // the EOS schedule
+    const size_t n_node_arrays = 7, n_elem_arrays = 19, n_corner = 48;
+    size_t bytes = (n_node_arrays * pn + n_elem_arrays * pe + n_corner * pe) *
+                   sizeof(double);
+    const size_t off_sched = bytes;
+    bytes += round_up(pe * sizeof(int32_t), 256) + round_up(pe * sizeof(uint16_t), 256);
+    const size_t off_scalars = bytes;
+    bytes += round_up(sizeof(DevScalars), 256);
+    HIP_TRY(d, hipMalloc(&d->arena, bytes));
+    HIP_TRY(d, hipMemsetAsync(d->arena, 0, bytes, d->stream));
+
+    char *base = static_cast<char *>(d->arena);
+    double *cur = reinterpret_cast<double *>(base);
+    auto take_node = [&]() { double *p = cur; cur += pn; return p; };
+    auto take_elem = [&]() { double *p = cur; cur += pe; return p; };
+    DevFields &f = d->f;
+    f.nx = m.nx;
+    f.num_elem = m.num_elem;
+    f.num_node = m.num_node;
+    f.x = take_node(); f.y = take_node(); f.z = take_node();
+    f.xd = take_node(); f.yd = take_node(); f.zd = take_node();
+    f.nodal_mass = take_node();
+    f.e = take_elem(); f.p = take_elem(); f.q = take_elem();
+    f.ql = take_elem(); f.qq = take_elem(); f.v = take_elem();
+    f.volo = take_elem(); f.elem_mass = take_elem(); f.ss = take_elem();
+    f.vnew = take_elem(); f.delv = take_elem(); f.arealg = take_elem();
+    f.vdov = take_elem();
+    f.delv_xi = take_elem(); f.delv_eta = take_elem(); f.delv_zeta = take_elem();
+    f.delx_xi = take_elem(); f.delx_eta = take_elem(); f.delx_zeta = take_elem();
+    f.corner = cur;
+    int32_t *order = reinterpret_cast<int32_t *>(base + off_sched);
+    uint16_t *rep = reinterpret_cast<uint16_t *>(base + off_sched +
+                                                 round_up(pe * sizeof(int32_t), 256));
+    f.eos_order = order;
+    f.eos_rep = rep;
+    d->s = reinterpret_cast<DevScalars *>(base + off_scalars);
+
+    int rc;
+    if ((rc = copy_in(d, f.x, m.x.data(), m.num_node))) return rc;
+    if ((rc = copy_in(d, f.y, m.y.data(), m.num_node))) return rc;
+    if ((rc = copy_in(d, f.z, m.z.data(), m.num_node))) return rc;
+    if ((rc = copy_in(d, f.nodal_mass, m.nodal_mass.data(), m.num_node))) return rc;
+    if ((rc = copy_in(d, f.e, m.e.data(), m.num_elem))) return rc;
+    if ((rc = copy_in(d, f.volo, m.volo.data(), m.num_elem))) return rc;
+    if ((rc = copy_in(d, f.elem_mass, m.volo.data(), m.num_elem))) return rc;
+    {
+        std::vector<double> ones(ne, 1.0);
+        if ((rc = copy_in(d, f.v, ones.data(), m.num_elem))) return rc;
+        HIP_TRY(d, hipMemcpyAsync(order, m.eos_order.data(), ne * sizeof(int32_t),
+                                  hipMemcpyHostToDevice, d->stream));
+        HIP_TRY(d, hipMemcpyAsync(rep, m.eos_rep.data(), ne * sizeof(uint16_t),
+                                  hipMemcpyHostToDevice, d->stream));
+        HIP_TRY(d, hipStreamSynchronize(d->stream));
+    }
+
+    DevScalars s{};
+    s.time = 0.0;
+    s.deltatime = m.deltatime0;
+    s.dtcourant = 1.0e+20;
+    s.dthydro = 1.0e+20;
+    s.stoptime = d->cfg.stoptime;
+    s.dtfixed = m.dtfixed;
+    s.dtmax = 1.0e-2;
+    s.deltatimemultlb = 1.1;
+    s.deltatimemultub = 1.2;
+    double big = 1.0e+20;
+    std::memcpy(&s.acc_courant, &big, sizeof(big));
+    std::memcpy(&s.acc_hydro, &big, sizeof(big));
+    s.cycle = 0;
+    s.max_cycles = d->cfg.max_cycles;
+    s.active = 0;
+    s.error = 0;
+    return write_scalars(d, s);
+}
+
+PhysParams make_params(int ic_mode)
+{
+    PhysParams p{};
+    p.u_cut = 1.0e-7;
+    p.hgcoef = 3.0;
+    p.qstop = 1.0e+12;
+    p.monoq_max_slope = 1.0;
+    p.monoq_limiter_mult = 2.0;
+    p.e_cut = 1.0e-7;
+    p.p_cut = 1.0e-7;
+    p.q_cut = 1.0e-7;
+    p.v_cut = 1.0e-10;
+    p.qlc_monoq = 0.5;
+    p.qqc_monoq = 2.0 / 3.0;
+    p.qqc = 2.0;
+    p.eosvmax = 1.0e+9;
+    p.eosvmin = 1.0e-9;
+    p.pmin = 0.;
+    p.emin = -1.0e+15;
+    p.dvovmax = 0.1;
+    p.refdens = 1.0;
+    // sound-speed floor: reference/C/LULESH-OMP.cc:2453-2454 (1.0),
+    // lulesh-2.0.3/lulesh.cc:2113-2114 (2.0)
+    p.ss_tiny = (ic_mode == 1) ? .111111e-36 : .1111111e-36;
+    p.ss_floor = (ic_mode == 1) ? .333333e-18 : .3333333e-18;
+    p.ic_mode = ic_mode;
+    return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+void lulesh_config_default(lulesh_config *cfg)
+{
+    if (!cfg) {
+        return;
+    }
+    cfg->nx = 30;
+    cfg->ic_mode = LULESH_IC_2_0;
+    cfg->num_reg = 11;
+    cfg->balance = 1;
+    cfg->cost = 1;
+    cfg->max_cycles = 9999999;
+    cfg->stoptime = 1.0e-2;
+}
+
+int lulesh_create(const lulesh_config *cfg, lulesh_domain **out)
+{
+    if (!cfg || !out) {
+        return LULESH_BAD_ARG;
+    }
+    *out = nullptr;
+    if (!(cfg->stoptime > 0.0) || cfg->max_cycles < 0) {
+        return LULESH_BAD_ARG;
+    }
+    lulesh_domain *d = new (std::nothrow) lulesh_domain();
+    if (!d) {
+        return LULESH_BAD_ARG;
+    }
+    d->cfg = *cfg;
+    HostMesh m;
+    try {
+        MeshConfig mc;
+        mc.nx = cfg->nx;
+        mc.ic_mode = cfg->ic_mode;
+        mc.num_reg = cfg->num_reg;
+        mc.balance = cfg->balance;
+        mc.cost = cfg->cost;
+        m = build_mesh(mc);
+    } catch (const std::exception &ex) {
+        delete d;
+        return LULESH_BAD_ARG;
+    }
+    d->nx = m.nx;
+    d->num_elem = m.num_elem;
+    d->num_node = m.num_node;
+    d->num_reg = m.num_reg;
+    d->reg_num_list = m.reg_num_list;
+    d->prm = make_params(cfg->ic_mode);
+    int rc = LULESH_OK;
+    hipError_t he = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) {
+        delete d;
+        return LULESH_HIP_ERROR;
+    }
+    if (hipEventCreate(&d->step_begin) != hipSuccess ||
+        hipEventCreate(&d->step_end) != hipSuccess) {
+        lulesh_destroy(d);
+        return LULESH_HIP_ERROR;
+    }
+    rc = build_device_domain(d, m);
+    if (rc != LULESH_OK) {
+        lulesh_destroy(d);
+        return rc;
+    }
+    *out = d;
+    return LULESH_OK;
+}
+
+void lulesh_destroy(lulesh_domain *d)
+{
+    if (!d) {
+        return;
+    }
+    if (d->stream) {
+        (void)hipStreamSynchronize(d->stream);
+    }
+    for (hipEvent_t ev : d->event_pool) {
+        (void)hipEventDestroy(ev);
+    }
+    if (d->step_begin) (void)hipEventDestroy(d->step_begin);
+    if (d->step_end) (void)hipEventDestroy(d->step_end);
+    if (d->arena) (void)hipFree(d->arena);
+    if (d->stream) (void)hipStreamDestroy(d->stream);
+    delete d;
+}
+
+int lulesh_step(lulesh_domain *d, int ncycles, int *cycles_done)
+{
+    if (cycles_done) {
+        *cycles_done = 0;
+    }
+    if (!d || ncycles < 0) {
+        return LULESH_BAD_ARG;
+    }
+    DevScalars before{}, after{};
+    int rc = read_scalars(d, &before);
+    if (rc) return rc;
+    if (before.error != 0) {
+        return fail(d, before.error, "domain is in an error state");
+    }
+    HIP_TRY(d, hipEventRecord(d->step_begin, d->stream));
+    for (int c = 0; c < ncycles; ++c) {
+        if ((rc = queue_cycle(d))) return rc;
+    }
+    HIP_TRY(d, hipEventRecord(d->step_end, d->stream));
+    rc = read_scalars(d, &after);
+    if (rc) return rc;
+    {
+        float ms = 0.f;
+        HIP_TRY(d, hipEventElapsedTime(&ms, d->step_begin, d->step_end));
+        d->last_step_ms = ms;
+    }
+    if (d->timing && (rc = collect_timings(d))) return rc;
+    if (cycles_done) {
+        *cycles_done = after.cycle - before.cycle;
+    }
+    if (after.error != 0) {
+        return fail(d, after.error,
+                    after.error == LULESH_VOLUME_ERROR ? "VolumeError" : "QStopError");
+    }
+    return LULESH_OK;
+}
+
+int lulesh_run(lulesh_domain *d, int *cycles_done)
+{
+    if (cycles_done) {
+        *cycles_done = 0;
+    }
+    if (!d) {
+        return LULESH_BAD_ARG;
+    }
+    for (;;) {
+        int n = 0;
+        const int rc = lulesh_step(d, kBatch, &n);
+        if (cycles_done) {
+            *cycles_done += n;
+        }
+        if (rc != LULESH_OK) {
+            return rc;
+        }
+        if (n < kBatch) {
+            return LULESH_OK;
+        }
+    }
+}
+
+int lulesh_upload_state(lulesh_domain *d, const double *x, const double *y,
+                        const double *z, const double *xd, const double *yd,
+                        const double *zd, const double *e, const double *p,
+                        const double *q, const double *v, const double *ss,
+                        double time, double deltatime)
+{
+    if (!d || !x || !y || !z || !xd || !yd || !zd || !e || !p || !q || !v || !ss) {
+        return LULESH_BAD_ARG;
+    }
+    const DevFields &f = d->f;
+    const int64_t nn = d->num_node, ne = d->num_elem;
+    int rc;
+    if ((rc = copy_in(d, f.x, x, nn)) || (rc = copy_in(d, f.y, y, nn)) ||
+        (rc = copy_in(d, f.z, z, nn)) || (rc = copy_in(d, f.xd, xd, nn)) ||
+        (rc = copy_in(d, f.yd, yd, nn)) || (rc = copy_in(d, f.zd, zd, nn)) ||
+        (rc = copy_in(d, f.e, e, ne)) || (rc = copy_in(d, f.p, p, ne)) ||
+        (rc = copy_in(d, f.q, q, ne)) || (rc = copy_in(d, f.v, v, ne)) ||
+        (rc = copy_in(d, f.ss, ss, ne)))
+        return rc;
+    return lulesh_set_scalars(d, &time, &deltatime, nullptr, nullptr, nullptr);
+}
+
+int lulesh_download_state(lulesh_domain *d, double *x, double *y, double *z,
+                          double *xd, double *yd, double *zd, double *e,
+                          double *p, double *q, double *v, double *ss,
+                          double *time, double *deltatime)
+{
+    if (!d || !x || !y || !z || !xd || !yd || !zd || !e || !p || !q || !v || !ss) {
+        return LULESH_BAD_ARG;
+    }
+    const DevFields &f = d->f;
+    const int64_t nn = d->num_node, ne = d->num_elem;
+    int rc;
+    if ((rc = copy_out(d, x, f.x, nn)) || (rc = copy_out(d, y, f.y, nn)) ||
+        (rc = copy_out(d, z, f.z, nn)) || (rc = copy_out(d, xd, f.xd, nn)) ||
+        (rc = copy_out(d, yd, f.yd, nn)) || (rc = copy_out(d, zd, f.zd, nn)) ||
+        (rc = copy_out(d, e, f.e, ne)) || (rc = copy_out(d, p, f.p, ne)) ||
+        (rc = copy_out(d, q, f.q, ne)) || (rc = copy_out(d, v, f.v, ne)) ||
+        (rc = copy_out(d, ss, f.ss, ne)))
+        return rc;
+    DevScalars s{};
+    if ((rc = read_scalars(d, &s))) return rc;
+    if (time) *time = s.time;
+    if (deltatime) *deltatime = s.deltatime;
+    return LULESH_OK;
+}
+
+int lulesh_get_field(lulesh_domain *d, int field, double *host_out)
+{
+    if (!d || !host_out) {
+        return LULESH_BAD_ARG;
+    }
+    const FieldRef r = field_ref(d, field);
+    if (!r.ptr) {
+        return fail(d, LULESH_BAD_ARG, "unknown field id");
+    }
+    int rc = copy_out(d, host_out, r.ptr, r.n);
+    if (rc) return rc;
+    HIP_TRY(d, hipStreamSynchronize(d->stream));
+    return LULESH_OK;
+}
+
+int lulesh_set_field(lulesh_domain *d, int field, const double *host_in)
+{
+    if (!d || !host_in) {
+        return LULESH_BAD_ARG;
+    }
+    const FieldRef r = field_ref(d, field);
+    if (!r.ptr) {
+        return fail(d, LULESH_BAD_ARG, "unknown field id");
+    }
+    int rc = copy_in(d, r.ptr, host_in, r.n);
+    if (rc) return rc;
+    HIP_TRY(d, hipStreamSynchronize(d->stream));
+    return LULESH_OK;
+}
+
+int lulesh_get_stats(lulesh_domain *d, lulesh_stats *out)
+{
+    if (!d || !out) {
+        return LULESH_BAD_ARG;
+    }
+    DevScalars s{};
+    int rc = read_scalars(d, &s);
+    if (rc) return rc;
+    double e0 = 0.0;
+    HIP_TRY(d, hipMemcpy(&e0, d->f.e, sizeof(double), hipMemcpyDeviceToHost));
+    out->time = s.time;
+    out->deltatime = s.deltatime;
+    out->dtcourant = s.dtcourant;
+    out->dthydro = s.dthydro;
+    out->e_origin = e0;
+    out->cycle = s.cycle;
+    out->done = (!(s.time < s.stoptime) || !(s.cycle < s.max_cycles)) ? 1 : 0;
+    out->num_elem = int(d->num_elem);
+    out->num_node = int(d->num_node);
+    return LULESH_OK;
+}
+
+int lulesh_set_scalars(lulesh_domain *d, const double *time,
+                       const double *deltatime, const double *dtcourant,
+                       const double *dthydro, const int *cycle)
+{
+    if (!d) {
+        return LULESH_BAD_ARG;
+    }
+    DevScalars s{};
+    int rc = read_scalars(d, &s);
+    if (rc) return rc;
+    if (time) s.time = *time;
+    if (deltatime) s.deltatime = *deltatime;
+    if (dtcourant) s.dtcourant = *dtcourant;
+    if (dthydro) s.dthydro = *dthydro;
+    if (cycle) s.cycle = *cycle;
+    s.error = 0;
+    return write_scalars(d, s);
+}
+
+int lulesh_get_regions(lulesh_domain *d, int *reg_num_list)
+{
+    if (!d || !reg_num_list) {
+        return LULESH_BAD_ARG;
+    }
+    std::memcpy(reg_num_list, d->reg_num_list.data(),
+                d->reg_num_list.size() * sizeof(int32_t));
+    return LULESH_OK;
+}
+
+int lulesh_symmetry(lulesh_domain *d, double *max_abs_diff,
+                    double *total_abs_diff, double *max_rel_diff)
+{
+    if (!d) {
+        return LULESH_BAD_ARG;
+    }
+    const int nx = d->nx;
+    std::vector<double> e(size_t(nx) * nx);
+    HIP_TRY(d, hipMemcpy(e.data(), d->f.e, e.size() * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    double mabs = 0.0, tabs = 0.0, mrel = 0.0;
+    for (int j = 0; j < nx; ++j) {
+        for (int k = j + 1; k < nx; ++k) {
+            const double a = std::fabs(e[size_t(j) * nx + k] - e[size_t(k) * nx + j]);
+            tabs += a;
+            if (mabs < a) mabs = a;
+            const double r = a / e[size_t(k) * nx + j];
+            if (mrel < r) mrel = r;
+        }
+    }
+    if (max_abs_diff) *max_abs_diff = mabs;
+    if (total_abs_diff) *total_abs_diff = tabs;
+    if (max_rel_diff) *max_rel_diff = mrel;
+    return LULESH_OK;
+}
+
+int lulesh_kernel_count(void) { return KID_COUNT; }
+
+const char *lulesh_kernel_name(int k)
+{
+    return (k >= 0 && k < KID_COUNT) ? kKernelNames[k] : "";
+}
+
+int lulesh_set_kernel_timing(lulesh_domain *d, int enable)
+{
+    if (!d) {
+        return LULESH_BAD_ARG;
+    }
+    d->timing = enable != 0;
+    for (int k = 0; k < KID_COUNT; ++k) {
+        d->kernel_ms[k] = 0.0;
+        d->kernel_launches[k] = 0;
+    }
+    return LULESH_OK;
+}
+
+int lulesh_kernel_times(lulesh_domain *d, double *ms, long long *launches)
+{
+    if (!d) {
+        return LULESH_BAD_ARG;
+    }
+    for (int k = 0; k < KID_COUNT; ++k) {
+        if (ms) ms[k] = d->kernel_ms[k];
+        if (launches) launches[k] = d->kernel_launches[k];
+    }
+    return LULESH_OK;
+}
+
+double lulesh_last_step_ms(lulesh_domain *d) { return d ? d->last_step_ms : 0.0; }
+
+const char *lulesh_last_error(const lulesh_domain *d)
+{
+    return d ? d->err.c_str() : "null domain";
+}
+
+int lulesh_device_math(int op, int n, const double *a, const double *b,
+                       double *out)
+{
+    if (n < 0 || !a || !out || (op == 2 && !b) || op < 0 || op > 2) {
+        return LULESH_BAD_ARG;
+    }
+    if (n == 0) {
+        return LULESH_OK;
+    }
+    double *da = nullptr, *db = nullptr, *dout = nullptr;
+    const size_t bytes = size_t(n) * sizeof(double);
+    int rc = LULESH_OK;
+    if (hipMalloc(&da, bytes) != hipSuccess || hipMalloc(&dout, bytes) != hipSuccess ||
+        (op == 2 && hipMalloc(&db, bytes) != hipSuccess)) {
+        rc = LULESH_HIP_ERROR;
+    }
+    if (rc == LULESH_OK && hipMemcpy(da, a, bytes, hipMemcpyHostToDevice) != hipSuccess)
+        rc = LULESH_HIP_ERROR;
+    if (rc == LULESH_OK && op == 2 &&
+        hipMemcpy(db, b, bytes, hipMemcpyHostToDevice) != hipSuccess)
+        rc = LULESH_HIP_ERROR;
+    if (rc == LULESH_OK) {
+        hipLaunchKernelGGL(k_device_math, dim3((n + 255) / 256), dim3(256), 0, 0,
+                           op, n, da, db, dout);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = LULESH_HIP_ERROR;
+    }
+    if (da) (void)hipFree(da);
+    if (db) (void)hipFree(db);
+    if (dout) (void)hipFree(dout);
+    return rc;
+}
+
+}  // extern "C"

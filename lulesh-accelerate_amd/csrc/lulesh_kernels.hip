@@ -1,0 +1,629 @@
+// lulesh_kernels.hip -- the phases of one LULESH cycle as gfx950 kernels.
+//
+// Cycle = TimeIncrement + LagrangeLeapFrog (reference/C/LULESH-OMP.cc:
+// 3179-3188; lulesh-2.0.3/lulesh.cc:2771-2781).  Kernels, in launch order:
+//
+//   k_time_increment   TimeIncrement, on the device (:557-604)
+//   k_corner_forces    per element: stress and hourglass corner forces
+//                      (:853-903, :1233-1473, :1504-1558)
+//   k_nodal_update     per node: ordered gather of the corner forces
+//                      (:905-926, :1475-1496), acceleration, symmetry BCs,
+//                      velocity and position (:1613-1704)
+//   k_kinematics_q     per element: kinematics and the monotonic-Q gradients
+//                      (:1927-2180)
+//   k_q_eos_dt         per element in region order: monotonic-Q limiter
+//                      (:2182-2320), qstop check (:2368-2381), EOS with the
+//                      region repetitions (:2384-2761; lulesh-2.0.3/lulesh.cc:
+//                      2357-2430), volume update (:2763-2782) and the
+//                      Courant / hydro minima (:2799-2942)
+//   k_dt_finish        applies the minima to dtcourant / dthydro
+//
+// Every kernel returns at once when the cycle is not active (the run is
+// over or an earlier phase raised an error), so whole batches of cycles can
+// be queued without a host round trip per cycle.
+#include "lulesh_kernels.hpp"
+
+#include "lulesh_elem.hpp"
+
+#pragma clang fp contract(off)
+
+namespace lulesh {
+
+namespace {
+
+constexpr double kBig = 1.0e+20;
+
+__device__ __forceinline__ unsigned long long as_bits(double v)
+{
+    return static_cast<unsigned long long>(__double_as_longlong(v));
+}
+
+__device__ __forceinline__ bool cycle_active(const DevScalars *s)
+{
+    return s->active != 0;
+}
+
+__device__ __forceinline__ void raise_error(DevScalars *s, int code)
+{
+    // the first error of a cycle wins; later phases see it and return
+    atomicCAS(&s->error, 0, code);
+}
+
+// minimum of a positive double over the block, folded into *acc with an
+// unsigned 64-bit atomic min on its bit pattern (ordered like the values for
+// non-negative doubles, so the result is exact and order independent)
+__device__ __forceinline__ void block_min_to(unsigned long long v,
+                                             unsigned long long *acc,
+                                             unsigned long long *lds)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off, 64);
+        v = (o < v) ? o : v;
+    }
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    if (lane == 0) {
+        lds[wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        unsigned long long m = lds[0];
+        for (int w = 1; w < nw; ++w) {
+            m = (lds[w] < m) ? lds[w] : m;
+        }
+        if (m < as_bits(kBig)) {
+            atomicMin(acc, m);
+        }
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// TimeIncrement (reference/C/LULESH-OMP.cc:557-604; lulesh-2.0.3/lulesh.cc:
+// 194-251).  Decides whether this cycle runs at all: the loop condition of
+// the drivers (time < stoptime, and cycle < its for 2.0) is evaluated here.
+// ---------------------------------------------------------------------------
+__global__ void k_time_increment(DevScalars *s)
+{
+    if (threadIdx.x != 0) {
+        return;
+    }
+    if (s->error != 0 || !(s->time < s->stoptime) || !(s->cycle < s->max_cycles)) {
+        s->active = 0;
+        return;
+    }
+    s->active = 1;
+    double targetdt = s->stoptime - s->time;
+    if ((s->dtfixed <= 0.0) && (s->cycle != 0)) {
+        double ratio;
+        const double olddt = s->deltatime;
+        double newdt = kBig;
+        if (s->dtcourant < newdt) {
+            newdt = s->dtcourant / 2.0;
+        }
+        if (s->dthydro < newdt) {
+            newdt = s->dthydro * 2.0 / 3.0;
+        }
+        ratio = newdt / olddt;
+        if (ratio >= 1.0) {
+            if (ratio < s->deltatimemultlb) {
+                newdt = olddt;
+            } else if (ratio > s->deltatimemultub) {
+                newdt = olddt * s->deltatimemultub;
+            }
+        }
+        if (newdt > s->dtmax) {
+            newdt = s->dtmax;
+        }
+        s->deltatime = newdt;
+    }
+    if ((targetdt > s->deltatime) && (targetdt < (4.0 * s->deltatime / 3.0))) {
+        targetdt = 2.0 * s->deltatime / 3.0;
+    }
+    if (targetdt < s->deltatime) {
+        s->deltatime = targetdt;
+    }
+    s->time += s->deltatime;
+    ++s->cycle;
+    s->acc_courant = as_bits(kBig);
+    s->acc_hydro = as_bits(kBig);
+}
+
+// ---------------------------------------------------------------------------
+// Corner forces of every element.  The stress part is IntegrateStressForElems
+// with InitStressTermsForElems (sigma = -p - q; f = -sigma * normal); the
+// hourglass part is CalcHourglassControlForElems + CalcFBHourglassForce with
+// determ = volo * v.  Corner c of element el is stored at
+// corner[(t * 8 + c) * num_elem + el], t = 0..2 stress x/y/z, 3..5 hourglass.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_corner_forces(DevFields f, DevScalars *s,
+                                                       PhysParams prm)
+{
+    if (!cycle_active(s) || s->error != 0) {
+        return;
+    }
+    const int64_t el = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (el >= f.num_elem) {
+        return;
+    }
+    const int nx = f.nx, en = f.nx + 1;
+    const int i = int(el % nx);
+    const int j = int((el / nx) % nx);
+    const int k = int(el / (int64_t(nx) * nx));
+    const int64_t n0 = (int64_t(k) * en + j) * en + i;
+    const int64_t plane = int64_t(en) * en;
+    const int64_t nodes[8] = {n0,         n0 + 1,         n0 + en + 1,         n0 + en,
+                              n0 + plane, n0 + plane + 1, n0 + plane + en + 1, n0 + plane + en};
+    double x[8], y[8], z[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        x[c] = f.x[nodes[c]];
+        y[c] = f.y[nodes[c]];
+        z[c] = f.z[nodes[c]];
+    }
+    const int64_t ne = f.num_elem;
+
+    // stress (reference/C/LULESH-OMP.cc:606-617, :864-903)
+    {
+        const double sig = -f.p[el] - f.q[el];
+        const double det = jacobian_volume(x, y, z);
+        if (det <= 0.0) {
+            raise_error(s, -1);
+        }
+        double pfx[8], pfy[8], pfz[8];
+        node_normals(pfx, pfy, pfz, x, y, z);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            f.corner[(0 * 8 + c) * ne + el] = -(sig * pfx[c]);
+            f.corner[(1 * 8 + c) * ne + el] = -(sig * pfy[c]);
+            f.corner[(2 * 8 + c) * ne + el] = -(sig * pfz[c]);
+        }
+    }
+
+    // hourglass (reference/C/LULESH-OMP.cc:1504-1558, :1291-1473)
+    {
+        const double vv = f.v[el];
+        if (vv <= 0.0) {
+            raise_error(s, -1);
+        }
+        const double determ = f.volo[el] * vv;
+        double hgx[8], hgy[8], hgz[8];
+        if (prm.hgcoef > 0.0) {
+            double dvdx[8], dvdy[8], dvdz[8], xd[8], yd[8], zd[8];
+            volume_derivative(dvdx, dvdy, dvdz, x, y, z);
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                xd[c] = f.xd[nodes[c]];
+                yd[c] = f.yd[nodes[c]];
+                zd[c] = f.zd[nodes[c]];
+            }
+            const double ss1 = f.ss[el];
+            const double mass1 = f.elem_mass[el];
+            const double volume13 = lulesh_cbrt(determ);
+            const double coefficient = -prm.hgcoef * 0.01 * ss1 * mass1 / volume13;
+            fb_hourglass(x, y, z, dvdx, dvdy, dvdz, xd, yd, zd, determ,
+                         coefficient, hgx, hgy, hgz);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                hgx[c] = hgy[c] = hgz[c] = 0.0;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            f.corner[(3 * 8 + c) * ne + el] = hgx[c];
+            f.corner[(4 * 8 + c) * ne + el] = hgy[c];
+            f.corner[(5 * 8 + c) * ne + el] = hgz[c];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Node update.  The forces of a node are gathered from its corners in
+// ascending element order -- the order of the reference's nodeElemCornerList
+// (reference/C/LULESH-OMP.cc:209-268) -- first the stress corners into
+// f = sum, then the hourglass corners into f += sum, each sum started from
+// 0.0 (:905-926, :1475-1496).  Then xdd = f / m, the symmetry planes zero
+// the normal acceleration, and velocity and position advance (:1613-1683).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_nodal_update(DevFields f, DevScalars *s,
+                                                      PhysParams prm)
+{
+    if (!cycle_active(s) || s->error != 0) {
+        return;
+    }
+    const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= f.num_node) {
+        return;
+    }
+    const int nx = f.nx, en = f.nx + 1;
+    const int I = int(n % en);
+    const int J = int((n / en) % en);
+    const int K = int(n / (int64_t(en) * en));
+    const int64_t ne = f.num_elem;
+
+    // the eight elements around the node in ascending index order, with the
+    // corner the node occupies in each: (dk, dj, di, corner)
+    constexpr int kAround[8][4] = {
+        {-1, -1, -1, 6}, {-1, -1, 0, 7}, {-1, 0, -1, 5}, {-1, 0, 0, 4},
+        {0, -1, -1, 2},  {0, -1, 0, 3},  {0, 0, -1, 1},  {0, 0, 0, 0},
+    };
+    double sum[6];
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        sum[t] = 0.0;
+    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const int ek = K + kAround[a][0], ej = J + kAround[a][1], ei = I + kAround[a][2];
+        if (ek < 0 || ek >= nx || ej < 0 || ej >= nx || ei < 0 || ei >= nx) {
+            continue;
+        }
+        const int64_t el = (int64_t(ek) * nx + ej) * nx + ei;
+        const int c = kAround[a][3];
+#pragma unroll
+        for (int t = 0; t < 6; ++t) {
+            sum[t] += f.corner[(t * 8 + c) * ne + el];
+        }
+    }
+    double fx = sum[0], fy = sum[1], fz = sum[2];
+    if (prm.hgcoef > 0.0) {
+        fx += sum[3];
+        fy += sum[4];
+        fz += sum[5];
+    }
+
+    const double mass = f.nodal_mass[n];
+    double xdd = fx / mass, ydd = fy / mass, zdd = fz / mass;
+    if (I == 0) xdd = 0.0;
+    if (J == 0) ydd = 0.0;
+    if (K == 0) zdd = 0.0;
+
+    const double dt = s->deltatime;
+    double xd = f.xd[n] + xdd * dt;
+    if (fabs(xd) < prm.u_cut) xd = 0.0;
+    double yd = f.yd[n] + ydd * dt;
+    if (fabs(yd) < prm.u_cut) yd = 0.0;
+    double zd = f.zd[n] + zdd * dt;
+    if (fabs(zd) < prm.u_cut) zd = 0.0;
+    f.xd[n] = xd;
+    f.yd[n] = yd;
+    f.zd[n] = zd;
+    f.x[n] = f.x[n] + xd * dt;
+    f.y[n] = f.y[n] + yd * dt;
+    f.z[n] = f.z[n] + zd * dt;
+}
+
+// ---------------------------------------------------------------------------
+// Kinematics (CalcKinematicsForElems + CalcLagrangeElements,
+// reference/C/LULESH-OMP.cc:1927-2032) and the monotonic-Q gradients
+// (CalcMonotonicQGradientsForElems, :2034-2180) of each element, sharing one
+// read of its eight nodes.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_kinematics_q(DevFields f, DevScalars *s,
+                                                      PhysParams prm)
+{
+    if (!cycle_active(s) || s->error != 0) {
+        return;
+    }
+    const int64_t el = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (el >= f.num_elem) {
+        return;
+    }
+    const int nx = f.nx, en = f.nx + 1;
+    const int i = int(el % nx);
+    const int j = int((el / nx) % nx);
+    const int k = int(el / (int64_t(nx) * nx));
+    const int64_t n0 = (int64_t(k) * en + j) * en + i;
+    const int64_t plane = int64_t(en) * en;
+    const int64_t nodes[8] = {n0,         n0 + 1,         n0 + en + 1,         n0 + en,
+                              n0 + plane, n0 + plane + 1, n0 + plane + en + 1, n0 + plane + en};
+    double x[8], y[8], z[8], xv[8], yv[8], zv[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        x[c] = f.x[nodes[c]];
+        y[c] = f.y[nodes[c]];
+        z[c] = f.z[nodes[c]];
+        xv[c] = f.xd[nodes[c]];
+        yv[c] = f.yd[nodes[c]];
+        zv[c] = f.zd[nodes[c]];
+    }
+    const double volo = f.volo[el];
+    const double dt = s->deltatime;
+
+    // kinematics
+    const double volume = elem_volume(x, y, z);
+    const double vnew = volume / volo;
+    f.vnew[el] = vnew;
+    f.delv[el] = vnew - f.v[el];
+    f.arealg[el] = characteristic_length(x, y, z, volume);
+    {
+        const double dt2 = 0.5 * dt;
+        double xh[8], yh[8], zh[8], b[3][4];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            xh[c] = x[c] - dt2 * xv[c];
+            yh[c] = y[c] - dt2 * yv[c];
+            zh[c] = z[c] - dt2 * zv[c];
+        }
+        const double det_j = shape_derivatives(xh, yh, zh, b);
+        const double inv_det_j = 1.0 / det_j;
+        const double dxx = inv_det_j * (b[0][0] * (xv[0] - xv[6]) + b[0][1] * (xv[1] - xv[7]) +
+                                        b[0][2] * (xv[2] - xv[4]) + b[0][3] * (xv[3] - xv[5]));
+        const double dyy = inv_det_j * (b[1][0] * (yv[0] - yv[6]) + b[1][1] * (yv[1] - yv[7]) +
+                                        b[1][2] * (yv[2] - yv[4]) + b[1][3] * (yv[3] - yv[5]));
+        const double dzz = inv_det_j * (b[2][0] * (zv[0] - zv[6]) + b[2][1] * (zv[1] - zv[7]) +
+                                        b[2][2] * (zv[2] - zv[4]) + b[2][3] * (zv[3] - zv[5]));
+        f.vdov[el] = dxx + dyy + dzz;
+    }
+    if (vnew <= 0.0) {
+        raise_error(s, -1);
+    }
+
+    // monotonic-Q gradients
+    const double ptiny = 1.e-36;
+    const double vol = volo * vnew;
+    const double norm = 1.0 / (vol + ptiny);
+    const double dxj = -0.25 * ((x[0] + x[1] + x[5] + x[4]) - (x[3] + x[2] + x[6] + x[7]));
+    const double dyj = -0.25 * ((y[0] + y[1] + y[5] + y[4]) - (y[3] + y[2] + y[6] + y[7]));
+    const double dzj = -0.25 * ((z[0] + z[1] + z[5] + z[4]) - (z[3] + z[2] + z[6] + z[7]));
+    const double dxi = 0.25 * ((x[1] + x[2] + x[6] + x[5]) - (x[0] + x[3] + x[7] + x[4]));
+    const double dyi = 0.25 * ((y[1] + y[2] + y[6] + y[5]) - (y[0] + y[3] + y[7] + y[4]));
+    const double dzi = 0.25 * ((z[1] + z[2] + z[6] + z[5]) - (z[0] + z[3] + z[7] + z[4]));
+    const double dxk = 0.25 * ((x[4] + x[5] + x[6] + x[7]) - (x[0] + x[1] + x[2] + x[3]));
+    const double dyk = 0.25 * ((y[4] + y[5] + y[6] + y[7]) - (y[0] + y[1] + y[2] + y[3]));
+    const double dzk = 0.25 * ((z[4] + z[5] + z[6] + z[7]) - (z[0] + z[1] + z[2] + z[3]));
+    double ax, ay, az, dxv, dyv, dzv;
+
+    ax = dyi * dzj - dzi * dyj;
+    ay = dzi * dxj - dxi * dzj;
+    az = dxi * dyj - dyi * dxj;
+    f.delx_zeta[el] = vol / sqrt(ax * ax + ay * ay + az * az + ptiny);
+    ax *= norm;
+    ay *= norm;
+    az *= norm;
+    dxv = 0.25 * ((xv[4] + xv[5] + xv[6] + xv[7]) - (xv[0] + xv[1] + xv[2] + xv[3]));
+    dyv = 0.25 * ((yv[4] + yv[5] + yv[6] + yv[7]) - (yv[0] + yv[1] + yv[2] + yv[3]));
+    dzv = 0.25 * ((zv[4] + zv[5] + zv[6] + zv[7]) - (zv[0] + zv[1] + zv[2] + zv[3]));
+    f.delv_zeta[el] = ax * dxv + ay * dyv + az * dzv;
+
+    ax = dyj * dzk - dzj * dyk;
+    ay = dzj * dxk - dxj * dzk;
+    az = dxj * dyk - dyj * dxk;
+    f.delx_xi[el] = vol / sqrt(ax * ax + ay * ay + az * az + ptiny);
+    ax *= norm;
+    ay *= norm;
+    az *= norm;
+    dxv = 0.25 * ((xv[1] + xv[2] + xv[6] + xv[5]) - (xv[0] + xv[3] + xv[7] + xv[4]));
+    dyv = 0.25 * ((yv[1] + yv[2] + yv[6] + yv[5]) - (yv[0] + yv[3] + yv[7] + yv[4]));
+    dzv = 0.25 * ((zv[1] + zv[2] + zv[6] + zv[5]) - (zv[0] + zv[3] + zv[7] + zv[4]));
+    f.delv_xi[el] = ax * dxv + ay * dyv + az * dzv;
+
+    ax = dyk * dzi - dzk * dyi;
+    ay = dzk * dxi - dxk * dzi;
+    az = dxk * dyi - dyk * dxi;
+    f.delx_eta[el] = vol / sqrt(ax * ax + ay * ay + az * az + ptiny);
+    ax *= norm;
+    ay *= norm;
+    az *= norm;
+    dxv = -0.25 * ((xv[0] + xv[1] + xv[5] + xv[4]) - (xv[3] + xv[2] + xv[6] + xv[7]));
+    dyv = -0.25 * ((yv[0] + yv[1] + yv[5] + yv[4]) - (yv[3] + yv[2] + yv[6] + yv[7]));
+    dzv = -0.25 * ((zv[0] + zv[1] + zv[5] + zv[4]) - (zv[3] + zv[2] + zv[6] + zv[7]));
+    f.delv_eta[el] = ax * dxv + ay * dyv + az * dzv;
+}
+
+namespace {
+
+// limiter of one direction (reference/C/LULESH-OMP.cc:2201-2228)
+__device__ __forceinline__ double mono_phi(double self, double delvm,
+                                           double delvp, double limiter_mult,
+                                           double max_slope)
+{
+    const double ptiny = 1.e-36;
+    const double norm = 1. / (self + ptiny);
+    double phi;
+    delvm = delvm * norm;
+    delvp = delvp * norm;
+    phi = .5 * (delvm + delvp);
+    delvm *= limiter_mult;
+    delvp *= limiter_mult;
+    if (delvm < phi) phi = delvm;
+    if (delvp < phi) phi = delvp;
+    if (phi < 0.) phi = 0.;
+    if (phi > max_slope) phi = max_slope;
+    return phi;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Monotonic Q, EOS, volume update and time constraints of each element.
+// Threads walk the elements region by region (eos_order), so a wavefront
+// mostly shares one repetition count.  The neighbour across a face is the
+// adjacent element of the structured cube; x = 0, y = 0 and z = 0 faces are
+// symmetry planes (neighbour value = own value), the far faces free surfaces
+// (neighbour value = 0) (reference/C/LULESH-OMP.cc:3153-3171, :2204-2287).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_q_eos_dt(DevFields f, DevScalars *s,
+                                                  PhysParams prm)
+{
+    __shared__ unsigned long long lds_c[4], lds_h[4];
+    __shared__ int skip;
+    // other blocks may raise an error while this one runs: read the flags
+    // once per block so that every thread takes the same path to the
+    // barriers of the reduction
+    if (threadIdx.x == 0) {
+        skip = (!cycle_active(s) || s->error != 0) ? 1 : 0;
+    }
+    __syncthreads();
+    if (skip) {
+        return;
+    }
+    const int64_t slot = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    unsigned long long cand_c = as_bits(kBig), cand_h = as_bits(kBig);
+    if (slot < f.num_elem) {
+        const int el32 = f.eos_order[slot];
+        const int64_t el = el32;
+        const int rep = f.eos_rep[slot];
+        const int nx = f.nx;
+        const int i = int(el % nx);
+        const int j = int((el / nx) % nx);
+        const int k = int(el / (int64_t(nx) * nx));
+        const int64_t sx = 1, sy = nx, sz = int64_t(nx) * nx;
+
+        // monotonic Q limiter and ql / qq
+        const double dvxi = f.delv_xi[el], dveta = f.delv_eta[el], dvzeta = f.delv_zeta[el];
+        const double lm = prm.monoq_limiter_mult, ms = prm.monoq_max_slope;
+        const double phixi = mono_phi(dvxi, (i == 0) ? dvxi : f.delv_xi[el - sx],
+                                      (i == nx - 1) ? 0.0 : f.delv_xi[el + sx], lm, ms);
+        const double phieta = mono_phi(dveta, (j == 0) ? dveta : f.delv_eta[el - sy],
+                                       (j == nx - 1) ? 0.0 : f.delv_eta[el + sy], lm, ms);
+        const double phizeta = mono_phi(dvzeta, (k == 0) ? dvzeta : f.delv_zeta[el - sz],
+                                        (k == nx - 1) ? 0.0 : f.delv_zeta[el + sz], lm, ms);
+        const double vdov = f.vdov[el];
+        const double vnew = f.vnew[el];
+        const double volo = f.volo[el];
+        double qlin, qquad;
+        if (vdov > 0.) {
+            qlin = 0.;
+            qquad = 0.;
+        } else {
+            double dxi = dvxi * f.delx_xi[el];
+            double deta = dveta * f.delx_eta[el];
+            double dzeta = dvzeta * f.delx_zeta[el];
+            if (dxi > 0.) dxi = 0.;
+            if (deta > 0.) deta = 0.;
+            if (dzeta > 0.) dzeta = 0.;
+            const double rho = f.elem_mass[el] / (volo * vnew);
+            qlin = -prm.qlc_monoq * rho *
+                   (dxi * (1. - phixi) + deta * (1. - phieta) + dzeta * (1. - phizeta));
+            qquad = prm.qqc_monoq * rho *
+                    (dxi * dxi * (1. - phixi * phixi) + deta * deta * (1. - phieta * phieta) +
+                     dzeta * dzeta * (1. - phizeta * phizeta));
+        }
+        f.ql[el] = qlin;
+        f.qq[el] = qquad;
+
+        // qstop is tested on the q of the previous cycle (:2368-2381)
+        const double q_old = f.q[el];
+        if (q_old > prm.qstop) {
+            raise_error(s, -2);
+        }
+
+        // clamp the new relative volume (:2704-2753; 2.0 clamps in place)
+        double vnewc = vnew;
+        if (prm.eosvmin != 0.) {
+            if (vnewc < prm.eosvmin) vnewc = prm.eosvmin;
+        }
+        if (prm.eosvmax != 0.) {
+            if (vnewc > prm.eosvmax) vnewc = prm.eosvmax;
+        }
+        double vc = f.v[el];
+        if (prm.eosvmin != 0.) {
+            if (vc < prm.eosvmin) vc = prm.eosvmin;
+        }
+        if (prm.eosvmax != 0.) {
+            if (vc > prm.eosvmax) vc = prm.eosvmax;
+        }
+        if (vc <= 0.) {
+            raise_error(s, -1);
+        }
+
+        EosConst ec;
+        ec.e_cut = prm.e_cut;
+        ec.p_cut = prm.p_cut;
+        ec.q_cut = prm.q_cut;
+        ec.eosvmax = prm.eosvmax;
+        ec.eosvmin = prm.eosvmin;
+        ec.pmin = prm.pmin;
+        ec.emin = prm.emin;
+        ec.rho0 = prm.refdens;
+        ec.ss_tiny = prm.ss_tiny;
+        ec.ss_floor = prm.ss_floor;
+        const double e_old = f.e[el], delvc = f.delv[el], p_old = f.p[el];
+        EosOut o;
+        o.p = o.e = o.q = o.ss = 0.0;
+        // the region's repetitions redo the same evaluation, as the
+        // reference does (lulesh-2.0.3/lulesh.cc:2267); the empty asm hides
+        // that the inputs are unchanged so every repetition is computed
+        for (int r = 0; r < rep; ++r) {
+            double a0 = vnewc, a1 = e_old, a2 = delvc, a3 = p_old, a4 = q_old;
+            double a5 = qquad, a6 = qlin;
+            asm volatile("" : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(a4),
+                         "+v"(a5), "+v"(a6));
+            o = eos_element(a0, a1, a2, a3, a4, a5, a6, ec);
+        }
+        f.p[el] = o.p;
+        f.e[el] = o.e;
+        f.q[el] = o.q;
+        f.ss[el] = o.ss;
+
+        // UpdateVolumesForElems: 1.0 uses the unclamped vnew, 2.0 the
+        // clamped one (lulesh-2.0.3/lulesh.cc:2439-2452)
+        double tmpv = (prm.ic_mode == 1) ? vnew : vnewc;
+        if (fabs(tmpv - 1.0) < prm.v_cut) tmpv = 1.0;
+        f.v[el] = tmpv;
+        if (prm.ic_mode == 2) {
+            f.vnew[el] = vnewc;
+        }
+
+        // Courant and hydro constraints (:2799-2933)
+        if (vdov != 0.) {
+            const double arealg = f.arealg[el];
+            const double qqc2 = 64.0 * prm.qqc * prm.qqc;
+            double dtf = o.ss * o.ss;
+            if (vdov < 0.) {
+                dtf = dtf + qqc2 * arealg * arealg * vdov * vdov;
+            }
+            dtf = sqrt(dtf);
+            dtf = arealg / dtf;
+            cand_c = as_bits(dtf);
+            const double dtdvov = prm.dvovmax / (fabs(vdov) + 1.e-20);
+            cand_h = as_bits(dtdvov);
+        }
+    }
+    block_min_to(cand_c, &s->acc_courant, lds_c);
+    block_min_to(cand_h, &s->acc_hydro, lds_h);
+}
+
+// ---------------------------------------------------------------------------
+// The minima replace dtcourant / dthydro when some element produced one
+// below 1e20; 2.0 starts every cycle from 1e20 (lulesh-2.0.3/lulesh.cc:
+// 2616-2620), 1.0 keeps the previous value otherwise (:2867-2871).
+// ---------------------------------------------------------------------------
+__global__ void k_dt_finish(DevScalars *s, int ic_mode)
+{
+    if (threadIdx.x != 0 || !cycle_active(s) || s->error != 0) {
+        return;
+    }
+    const double c = __longlong_as_double(static_cast<long long>(s->acc_courant));
+    const double h = __longlong_as_double(static_cast<long long>(s->acc_hydro));
+    if (ic_mode == 2) {
+        s->dtcourant = kBig;
+        s->dthydro = kBig;
+    }
+    if (c < kBig) s->dtcourant = c;
+    if (h < kBig) s->dthydro = h;
+}
+
+// device math probe: op 0 = lulesh_cbrt(a), 1 = sqrt(a), 2 = a / b
+__global__ void k_device_math(int op, int n, const double *a, const double *b,
+                              double *out)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) {
+        return;
+    }
+    if (op == 0) {
+        out[t] = lulesh_cbrt(a[t]);
+    } else if (op == 1) {
+        out[t] = sqrt(a[t]);
+    } else {
+        out[t] = a[t] / b[t];
+    }
+}
+
+}  // namespace lulesh

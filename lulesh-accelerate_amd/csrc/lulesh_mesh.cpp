@@ -1,0 +1,266 @@
+// lulesh_mesh.cpp -- host-side construction of the Sedov domain.
+//
+// The arithmetic that produces coordinates, volumes, masses and the initial
+// time step keeps the reference's operation order, so the uploaded state is
+// bit-identical to the reference's initial state.
+#include "lulesh_mesh.hpp"
+
+#include <cmath>
+#include <stdexcept>
+
+namespace lulesh {
+
+GlibcRand::GlibcRand(unsigned seed)
+{
+    // glibc srandom_r for TYPE_3 (degree 31, separation 3): Park-Miller
+    // seeding of 31 words, then 310 discarded outputs.  srand(0) seeds 1.
+    int32_t r[31];
+    if (seed == 0) {
+        seed = 1;
+    }
+    r[0] = static_cast<int32_t>(seed);
+    for (int i = 1; i < 31; ++i) {
+        const int64_t hi = r[i - 1] / 127773;
+        const int64_t lo = r[i - 1] % 127773;
+        int64_t w = 16807 * lo - 2836 * hi;
+        if (w < 0) {
+            w += 2147483647;
+        }
+        r[i] = static_cast<int32_t>(w);
+    }
+    // ring_[j] holds the value 31 outputs back of slot j: r[3..30], r[0..2]
+    for (int j = 0; j < 28; ++j) {
+        ring_[j] = static_cast<uint32_t>(r[j + 3]);
+    }
+    for (int j = 28; j < 31; ++j) {
+        ring_[j] = static_cast<uint32_t>(r[j - 28]);
+    }
+    pos_ = 0;
+    for (int i = 0; i < 310; ++i) {
+        next();
+    }
+}
+
+int GlibcRand::next()
+{
+    // r[n] = r[n-31] + r[n-3] (mod 2^32); rand() returns r[n] >> 1
+    const uint32_t v = ring_[pos_] + ring_[(pos_ + 28) % 31];
+    ring_[pos_] = v;
+    pos_ = (pos_ + 1) % 31;
+    return static_cast<int>(v >> 1);
+}
+
+int region_rep(int ic_mode, int num_reg, int cost, int r)
+{
+    if (ic_mode == 1) {
+        return 1;
+    }
+    if (r < num_reg / 2) {
+        return 1;
+    }
+    if (r < num_reg - (num_reg + 15) / 20) {
+        return 1 + cost;
+    }
+    return 10 * (1 + cost);
+}
+
+double hex_volume(const double x[8], const double y[8], const double z[8])
+{
+    auto tp = [](double x1, double y1, double z1, double x2, double y2,
+                 double z2, double x3, double y3, double z3) {
+        return x1 * (y2 * z3 - z2 * y3) + x2 * (z1 * y3 - y1 * z3) +
+               x3 * (y1 * z2 - z1 * y2);
+    };
+    const double twelveth = 1.0 / 12.0;
+    const double dx61 = x[6] - x[1], dy61 = y[6] - y[1], dz61 = z[6] - z[1];
+    const double dx70 = x[7] - x[0], dy70 = y[7] - y[0], dz70 = z[7] - z[0];
+    const double dx63 = x[6] - x[3], dy63 = y[6] - y[3], dz63 = z[6] - z[3];
+    const double dx20 = x[2] - x[0], dy20 = y[2] - y[0], dz20 = z[2] - z[0];
+    const double dx50 = x[5] - x[0], dy50 = y[5] - y[0], dz50 = z[5] - z[0];
+    const double dx64 = x[6] - x[4], dy64 = y[6] - y[4], dz64 = z[6] - z[4];
+    const double dx31 = x[3] - x[1], dy31 = y[3] - y[1], dz31 = z[3] - z[1];
+    const double dx72 = x[7] - x[2], dy72 = y[7] - y[2], dz72 = z[7] - z[2];
+    const double dx43 = x[4] - x[3], dy43 = y[4] - y[3], dz43 = z[4] - z[3];
+    const double dx57 = x[5] - x[7], dy57 = y[5] - y[7], dz57 = z[5] - z[7];
+    const double dx14 = x[1] - x[4], dy14 = y[1] - y[4], dz14 = z[1] - z[4];
+    const double dx25 = x[2] - x[5], dy25 = y[2] - y[5], dz25 = z[2] - z[5];
+    double v = tp(dx31 + dx72, dx63, dx20, dy31 + dy72, dy63, dy20,
+                  dz31 + dz72, dz63, dz20) +
+               tp(dx43 + dx57, dx64, dx70, dy43 + dy57, dy64, dy70,
+                  dz43 + dz57, dz64, dz70) +
+               tp(dx14 + dx25, dx61, dx50, dy14 + dy25, dy61, dy50,
+                  dz14 + dz25, dz61, dz50);
+    v *= twelveth;
+    return v;
+}
+
+namespace {
+
+// Region of every element: runs of 1..2048 elements drawn from the seeded
+// generator, the region of a run chosen with weight (r+1)^balance and never
+// equal to the previous run's region (lulesh-2.0.3/lulesh-init.cc:377-484).
+void assign_regions(HostMesh &m, int balance)
+{
+    const int64_t ne = m.num_elem;
+    const int nr = m.num_reg;
+    m.reg_num_list.assign(static_cast<size_t>(ne), 1);
+    if (nr == 1) {
+        return;
+    }
+    std::vector<int> bin_end(static_cast<size_t>(nr));
+    int cost_den = 0;
+    for (int i = 0; i < nr; ++i) {
+        cost_den += static_cast<int>(std::pow(double(i + 1), double(balance)));
+        bin_end[static_cast<size_t>(i)] = cost_den;
+    }
+    // run-length classes: (threshold on rand()%1000, modulus, offset)
+    struct RunClass { int below; int mod; int base; };
+    static const RunClass classes[] = {
+        {773, 15, 1},   {937, 16, 16},   {970, 32, 32},     {974, 64, 64},
+        {978, 128, 128}, {981, 256, 256}, {1000, 1537, 512},
+    };
+    GlibcRand g(0);
+    auto pick_region = [&]() {
+        const int var = g.next() % cost_den;
+        int i = 0;
+        while (var >= bin_end[static_cast<size_t>(i)]) {
+            ++i;
+        }
+        return (i % nr) + 1;
+    };
+    int last = -1;
+    int64_t next = 0;
+    while (next < ne) {
+        int reg = pick_region();
+        while (reg == last) {
+            reg = pick_region();
+        }
+        const int bin = g.next() % 1000;
+        int len = 0;
+        for (const RunClass &c : classes) {
+            if (bin < c.below) {
+                len = g.next() % c.mod + c.base;
+                break;
+            }
+        }
+        const int64_t end = next + len;
+        while (next < end && next < ne) {
+            m.reg_num_list[static_cast<size_t>(next++)] = reg;
+        }
+        last = reg;
+    }
+}
+
+}  // namespace
+
+HostMesh build_mesh(const MeshConfig &cfg)
+{
+    if (cfg.nx < 1 || cfg.nx > 600) {
+        throw std::invalid_argument("nx must be in [1, 600]");
+    }
+    if (cfg.ic_mode != 1 && cfg.ic_mode != 2) {
+        throw std::invalid_argument("ic_mode must be 1 or 2");
+    }
+    if (cfg.ic_mode == 2 && (cfg.num_reg < 1 || cfg.num_reg > 4096 || cfg.cost < 0 || cfg.cost > 1000 || cfg.balance < 0 || cfg.balance > 16)) {
+        throw std::invalid_argument("need 1 <= num_reg <= 4096, 0 <= cost <= 1000, 0 <= balance <= 16");
+    }
+    HostMesh m;
+    const int nx = cfg.nx, en = nx + 1;
+    m.nx = nx;
+    m.edge_nodes = en;
+    m.num_elem = int64_t(nx) * nx * nx;
+    m.num_node = int64_t(en) * en * en;
+    const size_t ne = static_cast<size_t>(m.num_elem);
+    const size_t nn = static_cast<size_t>(m.num_node);
+
+    // nodal coordinates (reference/C/LULESH-OMP.cc:2996-3017)
+    m.x.resize(nn);
+    m.y.resize(nn);
+    m.z.resize(nn);
+    std::vector<double> coord(static_cast<size_t>(en));
+    coord[0] = 0.0;
+    for (int c = 1; c < en; ++c) {
+        coord[static_cast<size_t>(c)] = 1.125 * double(c) / double(nx);
+    }
+    for (int k = 0; k < en; ++k) {
+        for (int j = 0; j < en; ++j) {
+            for (int i = 0; i < en; ++i) {
+                const size_t n = (size_t(k) * en + j) * en + i;
+                m.x[n] = coord[static_cast<size_t>(i)];
+                m.y[n] = coord[static_cast<size_t>(j)];
+                m.z[n] = coord[static_cast<size_t>(k)];
+            }
+        }
+    }
+
+    // element volumes, masses and nodal masses, in element order
+    // (reference/C/LULESH-OMP.cc:3089-3109)
+    m.volo.resize(ne);
+    m.nodal_mass.assign(nn, 0.0);
+    for (int k = 0; k < nx; ++k) {
+        for (int j = 0; j < nx; ++j) {
+            for (int i = 0; i < nx; ++i) {
+                const size_t el = (size_t(k) * nx + j) * nx + i;
+                const size_t n0 = (size_t(k) * en + j) * en + i;
+                const size_t nodes[8] = {
+                    n0, n0 + 1, n0 + en + 1, n0 + en,
+                    n0 + size_t(en) * en, n0 + size_t(en) * en + 1,
+                    n0 + size_t(en) * en + en + 1, n0 + size_t(en) * en + en};
+                double xl[8], yl[8], zl[8];
+                for (int c = 0; c < 8; ++c) {
+                    xl[c] = m.x[nodes[c]];
+                    yl[c] = m.y[nodes[c]];
+                    zl[c] = m.z[nodes[c]];
+                }
+                const double vol = hex_volume(xl, yl, zl);
+                m.volo[el] = vol;
+                for (int c = 0; c < 8; ++c) {
+                    m.nodal_mass[nodes[c]] += vol / 8.0;
+                }
+            }
+        }
+    }
+
+    // initial energy and time step
+    m.e.assign(ne, 0.0);
+    if (cfg.ic_mode == 1) {
+        // reference/C/LULESH-OMP.cc:3052-3053, :3112
+        m.e[0] = 3.948746e+7;
+        m.deltatime0 = 1.0e-7;
+        m.dtfixed = -1.0e-7;
+    } else {
+        // reference/C/lulesh-2.0.3/lulesh-init.cc:140, :167-182
+        const double ebase = 3.948746e+7;
+        const double scale = double(nx * 1) / 45.0;
+        const double einit = ebase * scale * scale * scale;
+        m.e[0] = einit;
+        m.deltatime0 = (.5 * std::cbrt(m.volo[0])) / std::sqrt(2.0 * einit);
+        m.dtfixed = -1.0e-6;
+    }
+
+    // regions and the EOS evaluation order
+    m.num_reg = (cfg.ic_mode == 1) ? 1 : cfg.num_reg;
+    assign_regions(m, cfg.balance);
+    m.reg_elem_size.assign(static_cast<size_t>(m.num_reg), 0);
+    for (size_t i = 0; i < ne; ++i) {
+        m.reg_elem_size[static_cast<size_t>(m.reg_num_list[i] - 1)]++;
+    }
+    m.region_offset.assign(static_cast<size_t>(m.num_reg) + 1, 0);
+    for (int r = 0; r < m.num_reg; ++r) {
+        m.region_offset[static_cast<size_t>(r) + 1] =
+            m.region_offset[static_cast<size_t>(r)] + m.reg_elem_size[static_cast<size_t>(r)];
+    }
+    m.eos_order.resize(ne);
+    m.eos_rep.resize(ne);
+    std::vector<int32_t> fill(m.region_offset.begin(), m.region_offset.end() - 1);
+    for (size_t i = 0; i < ne; ++i) {
+        const int r = m.reg_num_list[i] - 1;
+        const int32_t slot = fill[static_cast<size_t>(r)]++;
+        m.eos_order[static_cast<size_t>(slot)] = static_cast<int32_t>(i);
+        m.eos_rep[static_cast<size_t>(slot)] = static_cast<uint16_t>(
+            region_rep(cfg.ic_mode, m.num_reg, cfg.cost, r));
+    }
+    return m;
+}
+
+}  // namespace lulesh

@@ -1,0 +1,282 @@
+"""Python binding of the MI355X LULESH step (the C ABI of
+include/lulesh_mi355x.h), through ctypes.
+
+This is the host-side mirror of the reference driver's view of the step:
+`Domain` is the state the Accelerate program threads through `simulate`
+(src/Main.hs:90-109, src/Domain.hs:11-21), and `step`/`run` are the
+reference's cycle loop (reference/C/lulesh-2.0.3/lulesh.cc:2771-2781).
+The library must be built (lulesh-accelerate_amd/build.py); there is no
+fallback: without the HIP library every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "liblulesh_mi355x.so")
+
+OK, VOLUME_ERROR, QSTOP_ERROR, BAD_ARG, HIP_ERROR = 0, -1, -2, -3, -4
+
+NODE_FIELDS = {"x": 0, "y": 1, "z": 2, "xd": 3, "yd": 4, "zd": 5,
+               "nodalMass": 6}
+ELEM_FIELDS = {"e": 16, "p": 17, "q": 18, "ql": 19, "qq": 20, "v": 21,
+               "volo": 22, "delv": 23, "vdov": 24, "arealg": 25, "ss": 26,
+               "elemMass": 27, "vnew": 28, "delv_xi": 29, "delv_eta": 30,
+               "delv_zeta": 31, "delx_xi": 32, "delx_eta": 33,
+               "delx_zeta": 34}
+
+
+class LuleshError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"LULESH error {code}: {msg}")
+        self.code = code
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ic_mode", ctypes.c_int),
+                ("num_reg", ctypes.c_int), ("balance", ctypes.c_int),
+                ("cost", ctypes.c_int), ("max_cycles", ctypes.c_int),
+                ("stoptime", ctypes.c_double)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("time", ctypes.c_double), ("deltatime", ctypes.c_double),
+                ("dtcourant", ctypes.c_double), ("dthydro", ctypes.c_double),
+                ("e_origin", ctypes.c_double), ("cycle", ctypes.c_int),
+                ("done", ctypes.c_int), ("num_elem", ctypes.c_int),
+                ("num_node", ctypes.c_int)]
+
+
+_lib = None
+
+# every symbol declared in include/lulesh_mi355x.h
+EXPORTS = (
+    "lulesh_config_default", "lulesh_create", "lulesh_destroy", "lulesh_step",
+    "lulesh_run", "lulesh_upload_state", "lulesh_download_state",
+    "lulesh_get_field", "lulesh_set_field", "lulesh_get_stats",
+    "lulesh_set_scalars", "lulesh_get_regions", "lulesh_symmetry",
+    "lulesh_kernel_count", "lulesh_kernel_name", "lulesh_set_kernel_timing",
+    "lulesh_kernel_times", "lulesh_last_step_ms", "lulesh_last_error",
+    "lulesh_device_math",
+)
+
+
+def lib() -> ctypes.CDLL:
+    """Loads the HIP library; raises when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with "
+            "`python lulesh-accelerate_amd/build.py` (no CPU fallback exists)")
+    L = ctypes.CDLL(LIB_PATH)
+    P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    DP = ctypes.POINTER(ctypes.c_double)
+    IP = ctypes.POINTER(ctypes.c_int)
+    L.lulesh_config_default.argtypes = [ctypes.POINTER(Config)]
+    L.lulesh_config_default.restype = None
+    L.lulesh_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(P)]
+    L.lulesh_create.restype = I
+    L.lulesh_destroy.argtypes = [P]
+    L.lulesh_destroy.restype = None
+    L.lulesh_step.argtypes = [P, I, IP]
+    L.lulesh_step.restype = I
+    L.lulesh_run.argtypes = [P, IP]
+    L.lulesh_run.restype = I
+    L.lulesh_upload_state.argtypes = [P] + [DP] * 11 + [D, D]
+    L.lulesh_upload_state.restype = I
+    L.lulesh_download_state.argtypes = [P] + [DP] * 13
+    L.lulesh_download_state.restype = I
+    L.lulesh_get_field.argtypes = [P, I, DP]
+    L.lulesh_get_field.restype = I
+    L.lulesh_set_field.argtypes = [P, I, DP]
+    L.lulesh_set_field.restype = I
+    L.lulesh_get_stats.argtypes = [P, ctypes.POINTER(Stats)]
+    L.lulesh_get_stats.restype = I
+    L.lulesh_set_scalars.argtypes = [P, DP, DP, DP, DP, IP]
+    L.lulesh_set_scalars.restype = I
+    L.lulesh_get_regions.argtypes = [P, IP]
+    L.lulesh_get_regions.restype = I
+    L.lulesh_symmetry.argtypes = [P, DP, DP, DP]
+    L.lulesh_symmetry.restype = I
+    L.lulesh_kernel_count.argtypes = []
+    L.lulesh_kernel_count.restype = I
+    L.lulesh_kernel_name.argtypes = [I]
+    L.lulesh_kernel_name.restype = ctypes.c_char_p
+    L.lulesh_set_kernel_timing.argtypes = [P, I]
+    L.lulesh_set_kernel_timing.restype = I
+    L.lulesh_kernel_times.argtypes = [P, DP, ctypes.POINTER(ctypes.c_longlong)]
+    L.lulesh_kernel_times.restype = I
+    L.lulesh_last_step_ms.argtypes = [P]
+    L.lulesh_last_step_ms.restype = D
+    L.lulesh_last_error.argtypes = [P]
+    L.lulesh_last_error.restype = ctypes.c_char_p
+    L.lulesh_device_math.argtypes = [I, I, DP, DP, DP]
+    L.lulesh_device_math.restype = I
+    _lib = L
+    return L
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def default_config() -> Config:
+    c = Config()
+    lib().lulesh_config_default(ctypes.byref(c))
+    return c
+
+
+class Domain:
+    """A Sedov cube of nx^3 elements resident on the GPU."""
+
+    def __init__(self, nx: int = 30, ic_mode: int = 2, num_reg: int = 11,
+                 balance: int = 1, cost: int = 1, max_cycles: int = 9999999,
+                 stoptime: float = 1.0e-2):
+        L = lib()
+        cfg = default_config()
+        cfg.nx, cfg.ic_mode, cfg.num_reg = nx, ic_mode, num_reg
+        cfg.balance, cfg.cost = balance, cost
+        cfg.max_cycles, cfg.stoptime = max_cycles, stoptime
+        h = ctypes.c_void_p()
+        rc = L.lulesh_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != OK:
+            raise LuleshError(rc, "lulesh_create failed")
+        self.h = h
+        self.nx, self.ic_mode = nx, ic_mode
+        st = self.stats()
+        self.num_elem, self.num_node = st.num_elem, st.num_node
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().lulesh_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int) -> int:
+        if rc != OK:
+            msg = lib().lulesh_last_error(self.h).decode()
+            raise LuleshError(rc, msg)
+        return rc
+
+    def step(self, ncycles: int = 1, raise_on_error: bool = True) -> tuple[int, int]:
+        """Advances up to ncycles cycles; returns (rc, cycles done)."""
+        n = ctypes.c_int(0)
+        rc = lib().lulesh_step(self.h, int(ncycles), ctypes.byref(n))
+        if raise_on_error:
+            self._check(rc)
+        return rc, n.value
+
+    def run(self, raise_on_error: bool = True) -> tuple[int, int]:
+        n = ctypes.c_int(0)
+        rc = lib().lulesh_run(self.h, ctypes.byref(n))
+        if raise_on_error:
+            self._check(rc)
+        return rc, n.value
+
+    def get(self, name: str) -> np.ndarray:
+        if name in NODE_FIELDS:
+            fid, n = NODE_FIELDS[name], self.num_node
+        else:
+            fid, n = ELEM_FIELDS[name], self.num_elem
+        out = np.empty(n, np.float64)
+        self._check(lib().lulesh_get_field(self.h, fid, _dp(out)))
+        return out
+
+    def set(self, name: str, values) -> None:
+        if name in NODE_FIELDS:
+            fid, n = NODE_FIELDS[name], self.num_node
+        else:
+            fid, n = ELEM_FIELDS[name], self.num_elem
+        v = np.ascontiguousarray(values, np.float64)
+        if v.size != n:
+            raise ValueError(f"{name}: expected {n} values, got {v.size}")
+        self._check(lib().lulesh_set_field(self.h, fid, _dp(v)))
+
+    def stats(self) -> Stats:
+        s = Stats()
+        self._check(lib().lulesh_get_stats(self.h, ctypes.byref(s)))
+        return s
+
+    def set_scalars(self, time=None, deltatime=None, dtcourant=None,
+                    dthydro=None, cycle=None) -> None:
+        def dref(v):
+            return ctypes.byref(ctypes.c_double(v)) if v is not None else None
+        c = ctypes.byref(ctypes.c_int(int(cycle))) if cycle is not None else None
+        self._check(lib().lulesh_set_scalars(self.h, dref(time), dref(deltatime),
+                                             dref(dtcourant), dref(dthydro), c))
+
+    def regions(self) -> np.ndarray:
+        out = np.empty(self.num_elem, np.int32)
+        self._check(lib().lulesh_get_regions(
+            self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return out
+
+    def symmetry(self) -> dict:
+        a, t, r = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._check(lib().lulesh_symmetry(self.h, ctypes.byref(a),
+                                          ctypes.byref(t), ctypes.byref(r)))
+        return {"max_abs_diff": a.value, "total_abs_diff": t.value,
+                "max_rel_diff": r.value}
+
+    def download_state(self) -> dict:
+        nn, ne = self.num_node, self.num_elem
+        arr = {k: np.empty(nn) for k in ("x", "y", "z", "xd", "yd", "zd")}
+        arr.update({k: np.empty(ne) for k in ("e", "p", "q", "v", "ss")})
+        t, dt = ctypes.c_double(), ctypes.c_double()
+        order = ("x", "y", "z", "xd", "yd", "zd", "e", "p", "q", "v", "ss")
+        self._check(lib().lulesh_download_state(
+            self.h, *[_dp(arr[k]) for k in order], ctypes.byref(t),
+            ctypes.byref(dt)))
+        arr["time"], arr["deltatime"] = t.value, dt.value
+        return arr
+
+    def upload_state(self, st: dict) -> None:
+        order = ("x", "y", "z", "xd", "yd", "zd", "e", "p", "q", "v", "ss")
+        bufs = [np.ascontiguousarray(st[k], np.float64) for k in order]
+        self._check(lib().lulesh_upload_state(
+            self.h, *[_dp(b) for b in bufs], float(st["time"]),
+            float(st["deltatime"])))
+
+    # -- timing ---------------------------------------------------------
+    def set_kernel_timing(self, enable: bool) -> None:
+        self._check(lib().lulesh_set_kernel_timing(self.h, int(bool(enable))))
+
+    def kernel_times(self) -> dict:
+        k = lib().lulesh_kernel_count()
+        ms = (ctypes.c_double * k)()
+        n = (ctypes.c_longlong * k)()
+        self._check(lib().lulesh_kernel_times(self.h, ms, n))
+        return {lib().lulesh_kernel_name(i).decode(): (ms[i], n[i])
+                for i in range(k)}
+
+    def last_step_ms(self) -> float:
+        return lib().lulesh_last_step_ms(self.h)
+
+
+def device_math(op: str, a, b=None) -> np.ndarray:
+    """Evaluates the library's cube root, sqrt or division on the GPU."""
+    code = {"cbrt": 0, "sqrt": 1, "div": 2}[op]
+    a = np.ascontiguousarray(a, np.float64)
+    bb = np.ascontiguousarray(b, np.float64) if b is not None else None
+    out = np.empty_like(a)
+    rc = lib().lulesh_device_math(code, a.size, _dp(a),
+                                  _dp(bb) if bb is not None else None, _dp(out))
+    if rc != OK:
+        raise LuleshError(rc, "lulesh_device_math failed")
+    return out

@@ -1,0 +1,198 @@
+"""Parity of the HIP path with the oracle and the reference.
+
+The GPU step uses the project cube root in the hourglass coefficient, so it
+is compared bit for bit with the oracle run with that cube root
+(cbrt_mode=1), and within tolerance with the reference itself (golden dumps
+and published tables, which use the C library's cbrt).
+
+Tolerances against the reference: same cycle count,
+origin energy within 1e-10 relative, plane-0 symmetry MaxRelDiff <= 1e-12
+in 1.0 mode (2.0 mode: within 1e-12 of the reference's own value, which is
+1.461e-12 at 30^3), node and element fields within 1e-10 relative
+(max-norm of the difference over max-norm of the field).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+NODE = ("x", "y", "z", "xd", "yd", "zd", "nodalMass")
+ELEM = ("e", "p", "q", "ql", "qq", "v", "volo", "delv", "vdov", "arealg",
+        "ss", "elemMass")
+GRAD = ("delv_xi", "delv_eta", "delv_zeta", "delx_xi", "delx_eta", "delx_zeta")
+REL_TOL = 1e-10
+
+
+def bits_equal(a, b):
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    return np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+def rel_norm(a, b):
+    s = np.max(np.abs(b))
+    return 0.0 if s == 0 else float(np.max(np.abs(a - b)) / s)
+
+
+def assert_bitwise(dom, orc, fields):
+    bad = []
+    for f in fields:
+        g, o = dom.get(f), orc.get(f)
+        if not bits_equal(g, o):
+            n = int(np.count_nonzero(g.view(np.uint64) != o.view(np.uint64)))
+            bad.append((f, n, rel_norm(g, o)))
+    assert not bad, bad
+
+
+def make_pair(gpu, oracle, mode, nx, max_cycles=9999999, num_reg=11,
+              balance=1, cost=1):
+    d = gpu.Domain(nx, ic_mode=mode, num_reg=num_reg, balance=balance,
+                   cost=cost, max_cycles=max_cycles)
+    o = oracle.Oracle(nx, mode=mode, num_reg=num_reg, balance=balance,
+                      cost=cost, cbrt_mode=1, threads=16)
+    return d, o
+
+
+def test_device_math_matches_host(lulesh_gpu, oracle):
+    rng = np.random.default_rng(11)
+    a = np.ldexp(1 + rng.uniform(0, 1, 200000), rng.integers(-60, 60, 200000))
+    b = np.ldexp(1 + rng.uniform(0, 1, 200000), rng.integers(-60, 60, 200000))
+    L = oracle.lib()
+    host_cbrt = np.array([L.lo_cbrt_project(float(v)) for v in a[:20000]])
+    assert bits_equal(lulesh_gpu.device_math("cbrt", a[:20000]), host_cbrt)
+    assert bits_equal(lulesh_gpu.device_math("sqrt", a), np.sqrt(a))
+    assert bits_equal(lulesh_gpu.device_math("div", a, b), a / b)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_first_cycles_bitwise(lulesh_gpu, oracle, mode):
+    d, o = make_pair(lulesh_gpu, oracle, mode, 7)
+    for _ in range(3):
+        rc, n = d.step(1)
+        assert (rc, n) == (0, 1)
+        assert o.run(o.cycle + 1) == 0
+        st = d.stats()
+        assert st.cycle == o.cycle
+        assert st.time == o.scalar("time")
+        assert st.deltatime == o.scalar("deltatime")
+        assert st.dtcourant == o.scalar("dtcourant")
+        assert st.dthydro == o.scalar("dthydro")
+        assert_bitwise(d, o, NODE + ELEM + GRAD)
+    d.close()
+
+
+@pytest.mark.parametrize("mode,nx", [(1, 5), (2, 5), (1, 10), (2, 10)])
+def test_full_run_bitwise(lulesh_gpu, oracle, mode, nx):
+    d, o = make_pair(lulesh_gpu, oracle, mode, nx)
+    rc, n = d.run()
+    assert rc == 0
+    assert o.run() == 0
+    assert n == o.cycle
+    assert_bitwise(d, o, NODE + ELEM + GRAD)
+    d.close()
+
+
+def test_region_options_bitwise(lulesh_gpu, oracle):
+    d, o = make_pair(lulesh_gpu, oracle, 2, 12, max_cycles=60, num_reg=7,
+                     balance=2, cost=3)
+    assert d.run()[0] == 0
+    assert o.run(60) == 0
+    np.testing.assert_array_equal(d.regions(), o.get_int("regNumList"))
+    assert_bitwise(d, o, NODE + ELEM)
+    d.close()
+
+
+@pytest.mark.parametrize("name", ["ref20_s10", "ref20_s12_i60_r7_b2_c3",
+                                  "ref20_s16_i40"])
+def test_regions_match_reference(lulesh_gpu, name):
+    meta = json.load(open(os.path.join(GOLDEN, "reference_runs.json")))[name]
+    gold = np.load(os.path.join(GOLDEN, name + ".npz"))
+    opts = dict(zip(meta["extra"][0::2], meta["extra"][1::2]))
+    d = lulesh_gpu.Domain(meta["nx"], ic_mode=2, num_reg=int(opts.get("-r", 11)),
+                          balance=int(opts.get("-b", 1)),
+                          cost=int(opts.get("-c", 1)), max_cycles=0)
+    np.testing.assert_array_equal(d.regions(), gold["regNumList"])
+    d.close()
+
+
+@pytest.mark.parametrize("name", ["ref10_s5", "ref10_s10", "ref20_s5",
+                                  "ref20_s10", "ref20_s12_i60_r7_b2_c3",
+                                  "ref20_s16_i40"])
+def test_matches_reference_dumps_within_tolerance(lulesh_gpu, name):
+    """Against the reference's own final state (C-library cube root)."""
+    meta = json.load(open(os.path.join(GOLDEN, "reference_runs.json")))[name]
+    gold = np.load(os.path.join(GOLDEN, name + ".npz"))
+    opts = dict(zip(meta["extra"][0::2], meta["extra"][1::2]))
+    d = lulesh_gpu.Domain(meta["nx"], ic_mode=meta["mode"],
+                          num_reg=int(opts.get("-r", 11)),
+                          balance=int(opts.get("-b", 1)),
+                          cost=int(opts.get("-c", 1)),
+                          max_cycles=meta["its"] or 9999999)
+    assert d.run()[0] == 0
+    st = d.stats()
+    assert st.cycle == meta["cycle"]
+    e0 = d.get("e")[0]
+    assert abs(e0 - gold["e"][0]) <= REL_TOL * abs(gold["e"][0])
+    for f in NODE + ELEM:
+        assert rel_norm(d.get(f), gold[f]) <= REL_TOL, f
+    d.close()
+
+
+KNOWN = json.load(open(os.path.join(GOLDEN, "known_answers.json")))
+
+
+@pytest.mark.parametrize("mode,row", [
+    (1, r) for r in KNOWN["ic_1_0"] if r["nx"] <= 45] + [
+    (2, r) for r in KNOWN["ic_2_0"] if r["nx"] <= 45])
+def test_known_answers(lulesh_gpu, mode, row):
+    d = lulesh_gpu.Domain(row["nx"], ic_mode=mode)
+    rc, n = d.run()
+    assert rc == 0
+    st = d.stats()
+    assert st.cycle == row["cycles"]
+    assert f"{st.e_origin:12.6e}" == f"{row['origin_energy']:12.6e}"
+    sym = d.symmetry()
+    # published bound (reference/lulesh-ports.pdf section 6.2): <= 1e-8
+    assert sym["max_rel_diff"] <= 1e-8
+    assert sym["max_abs_diff"] <= 1e-8 * abs(st.e_origin) or row["nx"] > 10
+    d.close()
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_30_cubed_against_oracle(lulesh_gpu, oracle, mode):
+    """The full 30^3 run (config 1 of the benchmark set): bitwise against the
+    oracle with the project cube root, and the north-star bounds against
+    the oracle with the C library's cube root (= the reference)."""
+    d, o = make_pair(lulesh_gpu, oracle, mode, 30)
+    assert d.run()[0] == 0
+    assert o.run() == 0
+    assert d.stats().cycle == o.cycle
+    assert_bitwise(d, o, NODE + ELEM)
+    ref = oracle.Oracle(30, mode=mode, cbrt_mode=0, threads=16)
+    assert ref.run() == 0
+    assert ref.cycle == o.cycle
+    e0g, e0r = d.get("e")[0], ref.get("e")[0]
+    assert abs(e0g - e0r) <= REL_TOL * abs(e0r)
+    for f in NODE + ELEM:
+        assert rel_norm(d.get(f), ref.get(f)) <= REL_TOL, f
+    sym_g = d.symmetry()["max_rel_diff"]
+    sym_r = oracle.symmetry(ref.get("e"), 30)["max_rel_diff"]
+    if mode == 1:
+        assert sym_g <= 1e-12
+    else:
+        assert abs(sym_g - sym_r) <= 1e-12
+    d.close()
+
+
+def test_150_cubed_first_cycles_bitwise(lulesh_gpu, oracle):
+    """The benchmark configuration (150^3, 2.0 regions) for a few cycles."""
+    d, o = make_pair(lulesh_gpu, oracle, 2, 150, max_cycles=3)
+    assert d.run() == (0, 3)
+    assert o.run(3) == 0
+    np.testing.assert_array_equal(d.regions(), o.get_int("regNumList"))
+    assert_bitwise(d, o, NODE + ELEM)
+    d.close()

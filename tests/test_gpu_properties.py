@@ -1,0 +1,100 @@
+"""Size-independent properties of the HIP path at the benchmark size
+(150^3) and failure handling forced from the host."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float64).view(np.uint64)
+
+
+def test_runs_are_bit_reproducible(lulesh_gpu):
+    """No atomics touch floating-point data: two runs give identical bits."""
+    out = []
+    for _ in range(2):
+        d = lulesh_gpu.Domain(150, ic_mode=2, max_cycles=20)
+        assert d.run() == (0, 20)
+        out.append({f: d.get(f) for f in ("x", "xd", "e", "p", "q", "v", "ss")})
+        d.close()
+    for f in out[0]:
+        assert np.array_equal(bits(out[0][f]), bits(out[1][f])), f
+
+
+def test_sedov_symmetry_at_150(lulesh_gpu):
+    """The Sedov problem is symmetric under any permutation of x, y, z;
+    the discrete solution keeps it up to rounding."""
+    d = lulesh_gpu.Domain(150, ic_mode=2, max_cycles=60)
+    assert d.run() == (0, 60)
+    n = 150
+    e = d.get("e").reshape(n, n, n)  # [z][y][x]
+    scale = np.abs(e).max()
+    assert scale > 0
+    for perm in ((0, 2, 1), (1, 0, 2), (2, 1, 0)):
+        assert np.max(np.abs(e - e.transpose(perm))) <= 1e-10 * scale
+    x = d.get("x").reshape(n + 1, n + 1, n + 1)
+    y = d.get("y").reshape(n + 1, n + 1, n + 1)
+    assert np.max(np.abs(x - y.transpose(0, 2, 1))) <= 1e-12
+    d.close()
+
+
+def test_checkpoint_resume_is_exact(lulesh_gpu):
+    """Downloading the state and scalars and uploading them into a fresh
+    domain continues the run bit for bit."""
+    a = lulesh_gpu.Domain(24, ic_mode=2, max_cycles=80)
+    assert a.step(40) == (0, 40)
+    st = a.download_state()
+    sc = a.stats()
+    extra = {f: a.get(f) for f in ("ql", "qq")}
+    assert a.step(40) == (0, 40)
+    b = lulesh_gpu.Domain(24, ic_mode=2, max_cycles=80)
+    b.upload_state(st)
+    for f, v in extra.items():
+        b.set(f, v)
+    b.set_scalars(dtcourant=sc.dtcourant, dthydro=sc.dthydro, cycle=sc.cycle)
+    assert b.step(40) == (0, 40)
+    for f in ("x", "y", "z", "xd", "yd", "zd", "e", "p", "q", "v", "ss"):
+        assert np.array_equal(bits(a.get(f)), bits(b.get(f))), f
+    a.close()
+    b.close()
+
+
+def test_qstop_error(lulesh_gpu):
+    """q above qstop (1e12) stops the run with QStopError, as the reference
+    exits with QStopError (reference/C/LULESH-OMP.cc:2368-2381)."""
+    d = lulesh_gpu.Domain(8, ic_mode=2)
+    q = d.get("q")
+    q[17] = 2.0e12
+    d.set("q", q)
+    rc, n = d.step(5, raise_on_error=False)
+    assert rc == lulesh_gpu.QSTOP_ERROR
+    assert n == 1
+    # the domain stays in the error state
+    assert d.step(1, raise_on_error=False)[0] == lulesh_gpu.QSTOP_ERROR
+    d.close()
+
+
+def test_volume_error(lulesh_gpu):
+    """A non-positive relative volume stops the run with VolumeError
+    (reference/C/LULESH-OMP.cc:1541-1543)."""
+    d = lulesh_gpu.Domain(8, ic_mode=1)
+    v = d.get("v")
+    v[3] = -1.0
+    d.set("v", v)
+    rc, _ = d.step(2, raise_on_error=False)
+    assert rc == lulesh_gpu.VOLUME_ERROR
+    d.close()
+
+
+def test_tangled_mesh_volume_error(lulesh_gpu):
+    """Folding an element inside out makes its Jacobian volume negative."""
+    d = lulesh_gpu.Domain(6, ic_mode=2)
+    x = d.get("x")
+    en = 7
+    node = (2 * en + 2) * en + 2
+    x[node] += 0.5  # past its +x neighbour (spacing 0.1875)
+    d.set("x", x)
+    rc, _ = d.step(1, raise_on_error=False)
+    assert rc == lulesh_gpu.VOLUME_ERROR
+    d.close()
