@@ -38,10 +38,10 @@ HBM_PEAK_GBS = 8000.0
 # region index to the Q-region, EOS and dt phases), assigned to the kernel
 # that performs the phase.  Sum: 520 B/element + 296 B/node per cycle.
 KERNEL_BYTES = {
-    # stress + hourglass: p,q,v,volo,ss,elemMass + nodelist; reads x..zd
-    "k_corner_forces": (80, 48),
-    # force write fx,fy,fz (24) + acceleration/BC/velocity/position (128)
-    "k_nodal_update": (0, 152),
+    # force phase: p,q,v,volo,ss,elemMass + nodelist (80), read x..zd and
+    # write fx,fy,fz (72 per node); nodal phase: read f, m, xd, x and write
+    # xd, x (128 per node)
+    "k_force_sweep": (80, 200),
     # kinematics (80 + 48 node) + monotonic-Q gradients (96 + 48 node)
     "k_kinematics_q": (176, 96),
     # Q region 124+4, EOS 104+4, dt constraints 24+4

@@ -123,6 +123,22 @@ int lulesh_set_scalars(lulesh_domain *d, const double *time,
                        const double *deltatime, const double *dtcourant,
                        const double *dthydro, const int *cycle);
 
+/* Sets one constant of the method by its reference name: "hgcoef", "qstop",
+ * "monoq_max_slope", "monoq_limiter_mult", "qlc_monoq", "qqc_monoq", "qqc",
+ * "eosvmax", "eosvmin", "pmin", "emin", "dvovmax", "refdens", "e_cut",
+ * "p_cut", "q_cut", "u_cut", "v_cut", "stoptime", "dtmax", "dtfixed"
+ * (the Domain parameters of reference/C/LULESH-OMP.cc:351-380 and
+ * lulesh-2.0.3/lulesh-init.cc:19-41).  Takes effect at the next cycle. */
+int lulesh_set_parameter(lulesh_domain *d, const char *name, double value);
+
+/* Launch shape of the nodal-phase kernel: tx x ty element columns per block,
+ * nz node planes per block, and the waves per SIMD its register budget
+ * leaves room for.  Only the shapes the library was built with are accepted
+ * (LULESH_BAD_ARG otherwise); results do not depend on the shape. */
+int lulesh_set_sweep_shape(lulesh_domain *d, int tx, int ty, int nz, int waves);
+int lulesh_get_sweep_shape(lulesh_domain *d, int *tx, int *ty, int *nz,
+                           int *waves);
+
 /* Region of each element (1-based, the reference's regNumList). */
 int lulesh_get_regions(lulesh_domain *d, int *reg_num_list);
 

@@ -21,7 +21,8 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "liblulesh_mi355x.so")
 HARNESS = os.path.join(PKG, "lulesh_mi355x")
 
-HIP_SOURCES = ["lulesh_kernels.hip", "lulesh_abi.hip", "lulesh_mesh.cpp"]
+HIP_SOURCES = ["lulesh_kernels.hip", "lulesh_force.hip", "lulesh_abi.hip",
+               "lulesh_mesh.cpp"]
 HEADERS = ["lulesh_kernels.hpp", "lulesh_elem.hpp", "lulesh_cbrt.h",
            "lulesh_mesh.hpp"]
 

@@ -23,8 +23,7 @@ namespace {
 
 enum KernelId {
     KID_TIME_INCREMENT = 0,
-    KID_CORNER_FORCES,
-    KID_NODAL_UPDATE,
+    KID_FORCE_SWEEP,
     KID_KINEMATICS_Q,
     KID_Q_EOS_DT,
     KID_DT_FINISH,
@@ -32,9 +31,14 @@ enum KernelId {
 };
 
 const char *const kKernelNames[KID_COUNT] = {
-    "k_time_increment", "k_corner_forces", "k_nodal_update",
-    "k_kinematics_q",   "k_q_eos_dt",      "k_dt_finish",
+    "k_time_increment", "k_force_sweep", "k_kinematics_q", "k_q_eos_dt",
+    "k_dt_finish",
 };
+
+// default launch shape of the force sweep (32 x 8 element columns, 16 node
+// planes per block, register budget for 2 waves per SIMD);
+// lulesh_set_sweep_shape overrides it
+constexpr SweepShape kDefaultSweep = {32, 8, 16, 2};
 
 constexpr int kBlock = 256;
 constexpr int kBatch = 32;  // cycles queued per host round trip in run()
@@ -58,6 +62,7 @@ struct lulesh_domain {
     DevFields f{};
     DevScalars *s = nullptr;
     PhysParams prm{};
+    SweepShape sweep = kDefaultSweep;
     void *arena = nullptr;
     hipStream_t stream = nullptr;
 
@@ -129,22 +134,17 @@ int launch(lulesh_domain *d, int kid, F &&body)
 int queue_cycle(lulesh_domain *d)
 {
     const unsigned ge = unsigned((d->num_elem + kBlock - 1) / kBlock);
-    const unsigned gn = unsigned((d->num_node + kBlock - 1) / kBlock);
     hipStream_t st = d->stream;
     DevScalars *s = d->s;
     const DevFields &f = d->f;
     const PhysParams &p = d->prm;
     int rc;
     if ((rc = launch(d, KID_TIME_INCREMENT, [&] {
-             hipLaunchKernelGGL(k_time_increment, dim3(1), dim3(64), 0, st, s);
+             hipLaunchKernelGGL(k_time_increment, dim3(1), dim3(64), 0, st, s, p.ic_mode);
          })))
         return rc;
-    if ((rc = launch(d, KID_CORNER_FORCES, [&] {
-             hipLaunchKernelGGL(k_corner_forces, dim3(ge), dim3(kBlock), 0, st, f, s, p);
-         })))
-        return rc;
-    if ((rc = launch(d, KID_NODAL_UPDATE, [&] {
-             hipLaunchKernelGGL(k_nodal_update, dim3(gn), dim3(kBlock), 0, st, f, s, p);
+    if ((rc = launch(d, KID_FORCE_SWEEP, [&] {
+             (void)launch_force_sweep(d->sweep, f, s, p, st);
          })))
         return rc;
     if ((rc = launch(d, KID_KINEMATICS_Q, [&] {
@@ -153,10 +153,6 @@ int queue_cycle(lulesh_domain *d)
         return rc;
     if ((rc = launch(d, KID_Q_EOS_DT, [&] {
              hipLaunchKernelGGL(k_q_eos_dt, dim3(ge), dim3(kBlock), 0, st, f, s, p);
-         })))
-        return rc;
-    if ((rc = launch(d, KID_DT_FINISH, [&] {
-             hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(64), 0, st, s, p.ic_mode);
          })))
         return rc;
     return LULESH_OK;
@@ -197,17 +193,18 @@ struct FieldRef {
     int64_t n;
 };
 
-FieldRef field_ref(lulesh_domain *d, int id)
+// par: the copy of the node arrays that holds the current state (cycle & 1)
+FieldRef field_ref(lulesh_domain *d, int id, int par)
 {
     const DevFields &f = d->f;
     const int64_t ne = d->num_elem, nn = d->num_node;
     switch (id) {
-    case LULESH_F_X: return {f.x, nn};
-    case LULESH_F_Y: return {f.y, nn};
-    case LULESH_F_Z: return {f.z, nn};
-    case LULESH_F_XD: return {f.xd, nn};
-    case LULESH_F_YD: return {f.yd, nn};
-    case LULESH_F_ZD: return {f.zd, nn};
+    case LULESH_F_X: return {f.x[par], nn};
+    case LULESH_F_Y: return {f.y[par], nn};
+    case LULESH_F_Z: return {f.z[par], nn};
+    case LULESH_F_XD: return {f.xd[par], nn};
+    case LULESH_F_YD: return {f.yd[par], nn};
+    case LULESH_F_ZD: return {f.zd[par], nn};
     case LULESH_F_NODAL_MASS: return {f.nodal_mass, nn};
     case LULESH_F_E: return {f.e, ne};
     case LULESH_F_P: return {f.p, ne};
@@ -250,11 +247,10 @@ int build_device_domain(lulesh_domain *d, const HostMesh &m)
 {
     const size_t ne = size_t(m.num_elem), nn = size_t(m.num_node);
     const size_t pe = round_up(ne, 64), pn = round_up(nn, 64);
-    // arena: 7 node arrays, 19 element arrays, 48 corner arrays, scalars,
-    // the EOS schedule
-    const size_t n_node_arrays = 7, n_elem_arrays = 19, n_corner = 48;
-    size_t bytes = (n_node_arrays * pn + n_elem_arrays * pe + n_corner * pe) *
-                   sizeof(double);
+    // arena: 13 node arrays (two copies of x..zd, nodal mass), 19 element
+    // arrays, the EOS schedule, the scalars
+    const size_t n_node_arrays = 13, n_elem_arrays = 19;
+    size_t bytes = (n_node_arrays * pn + n_elem_arrays * pe) * sizeof(double);
     const size_t off_sched = bytes;
     bytes += round_up(pe * sizeof(int32_t), 256) + round_up(pe * sizeof(uint16_t), 256);
     const size_t off_scalars = bytes;
@@ -268,10 +264,13 @@ int build_device_domain(lulesh_domain *d, const HostMesh &m)
     auto take_elem = [&]() { double *p = cur; cur += pe; return p; };
     DevFields &f = d->f;
     f.nx = m.nx;
-    f.num_elem = m.num_elem;
-    f.num_node = m.num_node;
-    f.x = take_node(); f.y = take_node(); f.z = take_node();
-    f.xd = take_node(); f.yd = take_node(); f.zd = take_node();
+    f.num_elem = int(m.num_elem);
+    f.num_node = int(m.num_node);
+    f.div_nx = make_fastdiv(unsigned(m.nx));
+    for (int c = 0; c < 2; ++c) {
+        f.x[c] = take_node(); f.y[c] = take_node(); f.z[c] = take_node();
+        f.xd[c] = take_node(); f.yd[c] = take_node(); f.zd[c] = take_node();
+    }
     f.nodal_mass = take_node();
     f.e = take_elem(); f.p = take_elem(); f.q = take_elem();
     f.ql = take_elem(); f.qq = take_elem(); f.v = take_elem();
@@ -280,7 +279,6 @@ int build_device_domain(lulesh_domain *d, const HostMesh &m)
     f.vdov = take_elem();
     f.delv_xi = take_elem(); f.delv_eta = take_elem(); f.delv_zeta = take_elem();
     f.delx_xi = take_elem(); f.delx_eta = take_elem(); f.delx_zeta = take_elem();
-    f.corner = cur;
     int32_t *order = reinterpret_cast<int32_t *>(base + off_sched);
     uint16_t *rep = reinterpret_cast<uint16_t *>(base + off_sched +
                                                  round_up(pe * sizeof(int32_t), 256));
@@ -289,9 +287,9 @@ int build_device_domain(lulesh_domain *d, const HostMesh &m)
     d->s = reinterpret_cast<DevScalars *>(base + off_scalars);
 
     int rc;
-    if ((rc = copy_in(d, f.x, m.x.data(), m.num_node))) return rc;
-    if ((rc = copy_in(d, f.y, m.y.data(), m.num_node))) return rc;
-    if ((rc = copy_in(d, f.z, m.z.data(), m.num_node))) return rc;
+    if ((rc = copy_in(d, f.x[0], m.x.data(), m.num_node))) return rc;
+    if ((rc = copy_in(d, f.y[0], m.y.data(), m.num_node))) return rc;
+    if ((rc = copy_in(d, f.z[0], m.z.data(), m.num_node))) return rc;
     if ((rc = copy_in(d, f.nodal_mass, m.nodal_mass.data(), m.num_node))) return rc;
     if ((rc = copy_in(d, f.e, m.e.data(), m.num_elem))) return rc;
     if ((rc = copy_in(d, f.volo, m.volo.data(), m.num_elem))) return rc;
@@ -462,6 +460,13 @@ int lulesh_step(lulesh_domain *d, int ncycles, int *cycles_done)
     for (int c = 0; c < ncycles; ++c) {
         if ((rc = queue_cycle(d))) return rc;
     }
+    // the last cycle's time constraints (the others were applied by the
+    // following cycle's TimeIncrement)
+    if ((rc = launch(d, KID_DT_FINISH, [&] {
+             hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(64), 0, d->stream, d->s,
+                                d->prm.ic_mode);
+         })))
+        return rc;
     HIP_TRY(d, hipEventRecord(d->step_end, d->stream));
     rc = read_scalars(d, &after);
     if (rc) return rc;
@@ -515,10 +520,13 @@ int lulesh_upload_state(lulesh_domain *d, const double *x, const double *y,
     }
     const DevFields &f = d->f;
     const int64_t nn = d->num_node, ne = d->num_elem;
-    int rc;
-    if ((rc = copy_in(d, f.x, x, nn)) || (rc = copy_in(d, f.y, y, nn)) ||
-        (rc = copy_in(d, f.z, z, nn)) || (rc = copy_in(d, f.xd, xd, nn)) ||
-        (rc = copy_in(d, f.yd, yd, nn)) || (rc = copy_in(d, f.zd, zd, nn)) ||
+    DevScalars s{};
+    int rc = read_scalars(d, &s);
+    if (rc) return rc;
+    const int c = s.cur;
+    if ((rc = copy_in(d, f.x[c], x, nn)) || (rc = copy_in(d, f.y[c], y, nn)) ||
+        (rc = copy_in(d, f.z[c], z, nn)) || (rc = copy_in(d, f.xd[c], xd, nn)) ||
+        (rc = copy_in(d, f.yd[c], yd, nn)) || (rc = copy_in(d, f.zd[c], zd, nn)) ||
         (rc = copy_in(d, f.e, e, ne)) || (rc = copy_in(d, f.p, p, ne)) ||
         (rc = copy_in(d, f.q, q, ne)) || (rc = copy_in(d, f.v, v, ne)) ||
         (rc = copy_in(d, f.ss, ss, ne)))
@@ -536,16 +544,18 @@ int lulesh_download_state(lulesh_domain *d, double *x, double *y, double *z,
     }
     const DevFields &f = d->f;
     const int64_t nn = d->num_node, ne = d->num_elem;
-    int rc;
-    if ((rc = copy_out(d, x, f.x, nn)) || (rc = copy_out(d, y, f.y, nn)) ||
-        (rc = copy_out(d, z, f.z, nn)) || (rc = copy_out(d, xd, f.xd, nn)) ||
-        (rc = copy_out(d, yd, f.yd, nn)) || (rc = copy_out(d, zd, f.zd, nn)) ||
+    DevScalars s{};
+    int rc = read_scalars(d, &s);
+    if (rc) return rc;
+    const int c = s.cur;
+    if ((rc = copy_out(d, x, f.x[c], nn)) || (rc = copy_out(d, y, f.y[c], nn)) ||
+        (rc = copy_out(d, z, f.z[c], nn)) || (rc = copy_out(d, xd, f.xd[c], nn)) ||
+        (rc = copy_out(d, yd, f.yd[c], nn)) || (rc = copy_out(d, zd, f.zd[c], nn)) ||
         (rc = copy_out(d, e, f.e, ne)) || (rc = copy_out(d, p, f.p, ne)) ||
         (rc = copy_out(d, q, f.q, ne)) || (rc = copy_out(d, v, f.v, ne)) ||
         (rc = copy_out(d, ss, f.ss, ne)))
         return rc;
-    DevScalars s{};
-    if ((rc = read_scalars(d, &s))) return rc;
+    HIP_TRY(d, hipStreamSynchronize(d->stream));
     if (time) *time = s.time;
     if (deltatime) *deltatime = s.deltatime;
     return LULESH_OK;
@@ -556,11 +566,14 @@ int lulesh_get_field(lulesh_domain *d, int field, double *host_out)
     if (!d || !host_out) {
         return LULESH_BAD_ARG;
     }
-    const FieldRef r = field_ref(d, field);
+    DevScalars s{};
+    int rc = read_scalars(d, &s);
+    if (rc) return rc;
+    const FieldRef r = field_ref(d, field, s.cur);
     if (!r.ptr) {
         return fail(d, LULESH_BAD_ARG, "unknown field id");
     }
-    int rc = copy_out(d, host_out, r.ptr, r.n);
+    rc = copy_out(d, host_out, r.ptr, r.n);
     if (rc) return rc;
     HIP_TRY(d, hipStreamSynchronize(d->stream));
     return LULESH_OK;
@@ -571,11 +584,14 @@ int lulesh_set_field(lulesh_domain *d, int field, const double *host_in)
     if (!d || !host_in) {
         return LULESH_BAD_ARG;
     }
-    const FieldRef r = field_ref(d, field);
+    DevScalars s{};
+    int rc = read_scalars(d, &s);
+    if (rc) return rc;
+    const FieldRef r = field_ref(d, field, s.cur);
     if (!r.ptr) {
         return fail(d, LULESH_BAD_ARG, "unknown field id");
     }
-    int rc = copy_in(d, r.ptr, host_in, r.n);
+    rc = copy_in(d, r.ptr, host_in, r.n);
     if (rc) return rc;
     HIP_TRY(d, hipStreamSynchronize(d->stream));
     return LULESH_OK;
@@ -620,6 +636,71 @@ int lulesh_set_scalars(lulesh_domain *d, const double *time,
     if (cycle) s.cycle = *cycle;
     s.error = 0;
     return write_scalars(d, s);
+}
+
+int lulesh_set_parameter(lulesh_domain *d, const char *name, double value)
+{
+    if (!d || !name) {
+        return LULESH_BAD_ARG;
+    }
+    struct Entry { const char *name; double *field; };
+    PhysParams &p = d->prm;
+    const Entry table[] = {
+        {"hgcoef", &p.hgcoef}, {"qstop", &p.qstop},
+        {"monoq_max_slope", &p.monoq_max_slope},
+        {"monoq_limiter_mult", &p.monoq_limiter_mult},
+        {"qlc_monoq", &p.qlc_monoq}, {"qqc_monoq", &p.qqc_monoq},
+        {"qqc", &p.qqc}, {"eosvmax", &p.eosvmax}, {"eosvmin", &p.eosvmin},
+        {"pmin", &p.pmin}, {"emin", &p.emin}, {"dvovmax", &p.dvovmax},
+        {"refdens", &p.refdens}, {"e_cut", &p.e_cut}, {"p_cut", &p.p_cut},
+        {"q_cut", &p.q_cut}, {"u_cut", &p.u_cut}, {"v_cut", &p.v_cut},
+    };
+    for (const Entry &e : table) {
+        if (std::strcmp(e.name, name) == 0) {
+            *e.field = value;
+            return LULESH_OK;
+        }
+    }
+    // the remaining ones live with the device-resident loop scalars
+    DevScalars s{};
+    int rc = read_scalars(d, &s);
+    if (rc) return rc;
+    if (std::strcmp(name, "stoptime") == 0) {
+        s.stoptime = value;
+        d->cfg.stoptime = value;
+    } else if (std::strcmp(name, "dtmax") == 0) {
+        s.dtmax = value;
+    } else if (std::strcmp(name, "dtfixed") == 0) {
+        s.dtfixed = value;
+    } else {
+        return fail(d, LULESH_BAD_ARG, std::string("unknown parameter ") + name);
+    }
+    return write_scalars(d, s);
+}
+
+int lulesh_set_sweep_shape(lulesh_domain *d, int tx, int ty, int nz, int waves)
+{
+    if (!d) {
+        return LULESH_BAD_ARG;
+    }
+    const SweepShape sh = {tx, ty, nz, waves};
+    if (!force_sweep_shape_supported(sh)) {
+        return fail(d, LULESH_BAD_ARG, "sweep shape not built into the library");
+    }
+    d->sweep = sh;
+    return LULESH_OK;
+}
+
+int lulesh_get_sweep_shape(lulesh_domain *d, int *tx, int *ty, int *nz, int *waves)
+{
+    if (!d) {
+        return LULESH_BAD_ARG;
+    }
+    if (tx) *tx = d->sweep.tx;
+    if (ty) *ty = d->sweep.ty;
+    if (nz) *nz = d->sweep.nz;
+    if (waves) *waves = d->sweep.waves;
+    return LULESH_OK;
 }
 
 int lulesh_get_regions(lulesh_domain *d, int *reg_num_list)

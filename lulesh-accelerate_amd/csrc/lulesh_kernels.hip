@@ -1,22 +1,19 @@
-// lulesh_kernels.hip -- the phases of one LULESH cycle as gfx950 kernels.
+// lulesh_kernels.hip -- the element phases and the time step of a cycle.
 //
 // Cycle = TimeIncrement + LagrangeLeapFrog (reference/C/LULESH-OMP.cc:
 // 3179-3188; lulesh-2.0.3/lulesh.cc:2771-2781).  Kernels, in launch order:
 //
 //   k_time_increment   TimeIncrement, on the device (:557-604)
-//   k_corner_forces    per element: stress and hourglass corner forces
-//                      (:853-903, :1233-1473, :1504-1558)
-//   k_nodal_update     per node: ordered gather of the corner forces
-//                      (:905-926, :1475-1496), acceleration, symmetry BCs,
-//                      velocity and position (:1613-1704)
+//   k_force_sweep      LagrangeNodal: forces, acceleration, boundary
+//                      conditions, velocity, position (lulesh_force.hip)
 //   k_kinematics_q     per element: kinematics and the monotonic-Q gradients
 //                      (:1927-2180)
 //   k_q_eos_dt         per element in region order: monotonic-Q limiter
 //                      (:2182-2320), qstop check (:2368-2381), EOS with the
 //                      region repetitions (:2384-2761; lulesh-2.0.3/lulesh.cc:
 //                      2357-2430), volume update (:2763-2782) and the
-//                      Courant / hydro minima (:2799-2942)
-//   k_dt_finish        applies the minima to dtcourant / dthydro
+//                      Courant / hydro minima (:2799-2942), which the last
+//                      block to finish applies to dtcourant / dthydro
 //
 // Every kernel returns at once when the cycle is not active (the run is
 // over or an earlier phase raised an error), so whole batches of cycles can
@@ -38,9 +35,9 @@ __device__ __forceinline__ unsigned long long as_bits(double v)
     return static_cast<unsigned long long>(__double_as_longlong(v));
 }
 
-__device__ __forceinline__ bool cycle_active(const DevScalars *s)
+__device__ __forceinline__ double from_bits(unsigned long long b)
 {
-    return s->active != 0;
+    return __longlong_as_double(static_cast<long long>(b));
 }
 
 __device__ __forceinline__ void raise_error(DevScalars *s, int code)
@@ -49,34 +46,37 @@ __device__ __forceinline__ void raise_error(DevScalars *s, int code)
     atomicCAS(&s->error, 0, code);
 }
 
-// minimum of a positive double over the block, folded into *acc with an
-// unsigned 64-bit atomic min on its bit pattern (ordered like the values for
-// non-negative doubles, so the result is exact and order independent)
-__device__ __forceinline__ void block_min_to(unsigned long long v,
-                                             unsigned long long *acc,
-                                             unsigned long long *lds)
+// The Courant and hydro minima of a cycle are gathered in the accumulators
+// by k_q_eos_dt and applied here, at the next kernel boundary: by the next
+// cycle's TimeIncrement or by the k_dt_finish that closes a batch.
+__device__ __forceinline__ void apply_constraints(DevScalars *s, int ic_mode)
 {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        const unsigned long long o = __shfl_xor(v, off, 64);
-        v = (o < v) ? o : v;
+    if (!s->dt_pending) {
+        return;
     }
-    const int wave = threadIdx.x >> 6;
-    const int lane = threadIdx.x & 63;
-    if (lane == 0) {
-        lds[wave] = v;
+    s->dt_pending = 0;
+    if (s->error != 0) {
+        return;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int nw = (blockDim.x + 63) >> 6;
-        unsigned long long m = lds[0];
-        for (int w = 1; w < nw; ++w) {
-            m = (lds[w] < m) ? lds[w] : m;
-        }
-        if (m < as_bits(kBig)) {
-            atomicMin(acc, m);
-        }
+    const double c = from_bits(s->acc_courant);
+    const double h = from_bits(s->acc_hydro);
+    if (ic_mode == 2) {
+        s->dtcourant = kBig;
+        s->dthydro = kBig;
     }
+    if (c < kBig) s->dtcourant = c;
+    if (h < kBig) s->dthydro = h;
+}
+
+// element (i, j, k) of a flat element index
+__device__ __forceinline__ void elem_ijk(const DevFields &f, int el, int &i, int &j,
+                                         int &k)
+{
+    const unsigned q1 = fast_div(unsigned(el), f.div_nx);
+    i = el - int(q1) * f.nx;
+    const unsigned q2 = fast_div(q1, f.div_nx);
+    j = int(q1) - int(q2) * f.nx;
+    k = int(q2);
 }
 
 }  // namespace
@@ -86,11 +86,12 @@ __device__ __forceinline__ void block_min_to(unsigned long long v,
 // 194-251).  Decides whether this cycle runs at all: the loop condition of
 // the drivers (time < stoptime, and cycle < its for 2.0) is evaluated here.
 // ---------------------------------------------------------------------------
-__global__ void k_time_increment(DevScalars *s)
+__global__ void k_time_increment(DevScalars *s, int ic_mode)
 {
     if (threadIdx.x != 0) {
         return;
     }
+    apply_constraints(s, ic_mode);
     if (s->error != 0 || !(s->time < s->stoptime) || !(s->cycle < s->max_cycles)) {
         s->active = 0;
         return;
@@ -128,211 +129,54 @@ __global__ void k_time_increment(DevScalars *s)
     }
     s->time += s->deltatime;
     ++s->cycle;
+    s->cur ^= 1;  // this cycle's nodes go to the other copy
     s->acc_courant = as_bits(kBig);
     s->acc_hydro = as_bits(kBig);
-}
-
-// ---------------------------------------------------------------------------
-// Corner forces of every element.  The stress part is IntegrateStressForElems
-// with InitStressTermsForElems (sigma = -p - q; f = -sigma * normal); the
-// hourglass part is CalcHourglassControlForElems + CalcFBHourglassForce with
-// determ = volo * v.  Corner c of element el is stored at
-// corner[(t * 8 + c) * num_elem + el], t = 0..2 stress x/y/z, 3..5 hourglass.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_corner_forces(DevFields f, DevScalars *s,
-                                                       PhysParams prm)
-{
-    if (!cycle_active(s) || s->error != 0) {
-        return;
-    }
-    const int64_t el = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (el >= f.num_elem) {
-        return;
-    }
-    const int nx = f.nx, en = f.nx + 1;
-    const int i = int(el % nx);
-    const int j = int((el / nx) % nx);
-    const int k = int(el / (int64_t(nx) * nx));
-    const int64_t n0 = (int64_t(k) * en + j) * en + i;
-    const int64_t plane = int64_t(en) * en;
-    const int64_t nodes[8] = {n0,         n0 + 1,         n0 + en + 1,         n0 + en,
-                              n0 + plane, n0 + plane + 1, n0 + plane + en + 1, n0 + plane + en};
-    double x[8], y[8], z[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        x[c] = f.x[nodes[c]];
-        y[c] = f.y[nodes[c]];
-        z[c] = f.z[nodes[c]];
-    }
-    const int64_t ne = f.num_elem;
-
-    // stress (reference/C/LULESH-OMP.cc:606-617, :864-903)
-    {
-        const double sig = -f.p[el] - f.q[el];
-        const double det = jacobian_volume(x, y, z);
-        if (det <= 0.0) {
-            raise_error(s, -1);
-        }
-        double pfx[8], pfy[8], pfz[8];
-        node_normals(pfx, pfy, pfz, x, y, z);
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            f.corner[(0 * 8 + c) * ne + el] = -(sig * pfx[c]);
-            f.corner[(1 * 8 + c) * ne + el] = -(sig * pfy[c]);
-            f.corner[(2 * 8 + c) * ne + el] = -(sig * pfz[c]);
-        }
-    }
-
-    // hourglass (reference/C/LULESH-OMP.cc:1504-1558, :1291-1473)
-    {
-        const double vv = f.v[el];
-        if (vv <= 0.0) {
-            raise_error(s, -1);
-        }
-        const double determ = f.volo[el] * vv;
-        double hgx[8], hgy[8], hgz[8];
-        if (prm.hgcoef > 0.0) {
-            double dvdx[8], dvdy[8], dvdz[8], xd[8], yd[8], zd[8];
-            volume_derivative(dvdx, dvdy, dvdz, x, y, z);
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                xd[c] = f.xd[nodes[c]];
-                yd[c] = f.yd[nodes[c]];
-                zd[c] = f.zd[nodes[c]];
-            }
-            const double ss1 = f.ss[el];
-            const double mass1 = f.elem_mass[el];
-            const double volume13 = lulesh_cbrt(determ);
-            const double coefficient = -prm.hgcoef * 0.01 * ss1 * mass1 / volume13;
-            fb_hourglass(x, y, z, dvdx, dvdy, dvdz, xd, yd, zd, determ,
-                         coefficient, hgx, hgy, hgz);
-        } else {
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                hgx[c] = hgy[c] = hgz[c] = 0.0;
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            f.corner[(3 * 8 + c) * ne + el] = hgx[c];
-            f.corner[(4 * 8 + c) * ne + el] = hgy[c];
-            f.corner[(5 * 8 + c) * ne + el] = hgz[c];
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Node update.  The forces of a node are gathered from its corners in
-// ascending element order -- the order of the reference's nodeElemCornerList
-// (reference/C/LULESH-OMP.cc:209-268) -- first the stress corners into
-// f = sum, then the hourglass corners into f += sum, each sum started from
-// 0.0 (:905-926, :1475-1496).  Then xdd = f / m, the symmetry planes zero
-// the normal acceleration, and velocity and position advance (:1613-1683).
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_nodal_update(DevFields f, DevScalars *s,
-                                                      PhysParams prm)
-{
-    if (!cycle_active(s) || s->error != 0) {
-        return;
-    }
-    const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (n >= f.num_node) {
-        return;
-    }
-    const int nx = f.nx, en = f.nx + 1;
-    const int I = int(n % en);
-    const int J = int((n / en) % en);
-    const int K = int(n / (int64_t(en) * en));
-    const int64_t ne = f.num_elem;
-
-    // the eight elements around the node in ascending index order, with the
-    // corner the node occupies in each: (dk, dj, di, corner)
-    constexpr int kAround[8][4] = {
-        {-1, -1, -1, 6}, {-1, -1, 0, 7}, {-1, 0, -1, 5}, {-1, 0, 0, 4},
-        {0, -1, -1, 2},  {0, -1, 0, 3},  {0, 0, -1, 1},  {0, 0, 0, 0},
-    };
-    double sum[6];
-#pragma unroll
-    for (int t = 0; t < 6; ++t) {
-        sum[t] = 0.0;
-    }
-#pragma unroll
-    for (int a = 0; a < 8; ++a) {
-        const int ek = K + kAround[a][0], ej = J + kAround[a][1], ei = I + kAround[a][2];
-        if (ek < 0 || ek >= nx || ej < 0 || ej >= nx || ei < 0 || ei >= nx) {
-            continue;
-        }
-        const int64_t el = (int64_t(ek) * nx + ej) * nx + ei;
-        const int c = kAround[a][3];
-#pragma unroll
-        for (int t = 0; t < 6; ++t) {
-            sum[t] += f.corner[(t * 8 + c) * ne + el];
-        }
-    }
-    double fx = sum[0], fy = sum[1], fz = sum[2];
-    if (prm.hgcoef > 0.0) {
-        fx += sum[3];
-        fy += sum[4];
-        fz += sum[5];
-    }
-
-    const double mass = f.nodal_mass[n];
-    double xdd = fx / mass, ydd = fy / mass, zdd = fz / mass;
-    if (I == 0) xdd = 0.0;
-    if (J == 0) ydd = 0.0;
-    if (K == 0) zdd = 0.0;
-
-    const double dt = s->deltatime;
-    double xd = f.xd[n] + xdd * dt;
-    if (fabs(xd) < prm.u_cut) xd = 0.0;
-    double yd = f.yd[n] + ydd * dt;
-    if (fabs(yd) < prm.u_cut) yd = 0.0;
-    double zd = f.zd[n] + zdd * dt;
-    if (fabs(zd) < prm.u_cut) zd = 0.0;
-    f.xd[n] = xd;
-    f.yd[n] = yd;
-    f.zd[n] = zd;
-    f.x[n] = f.x[n] + xd * dt;
-    f.y[n] = f.y[n] + yd * dt;
-    f.z[n] = f.z[n] + zd * dt;
+    s->dt_pending = 1;
 }
 
 // ---------------------------------------------------------------------------
 // Kinematics (CalcKinematicsForElems + CalcLagrangeElements,
 // reference/C/LULESH-OMP.cc:1927-2032) and the monotonic-Q gradients
 // (CalcMonotonicQGradientsForElems, :2034-2180) of each element, sharing one
-// read of its eight nodes.
+// read of its eight advanced nodes.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_kinematics_q(DevFields f, DevScalars *s,
                                                       PhysParams prm)
 {
-    if (!cycle_active(s) || s->error != 0) {
+    if (s->active == 0 || s->error != 0) {
         return;
     }
-    const int64_t el = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int el = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
     if (el >= f.num_elem) {
         return;
     }
-    const int nx = f.nx, en = f.nx + 1;
-    const int i = int(el % nx);
-    const int j = int((el / nx) % nx);
-    const int k = int(el / (int64_t(nx) * nx));
-    const int64_t n0 = (int64_t(k) * en + j) * en + i;
-    const int64_t plane = int64_t(en) * en;
-    const int64_t nodes[8] = {n0,         n0 + 1,         n0 + en + 1,         n0 + en,
-                              n0 + plane, n0 + plane + 1, n0 + plane + en + 1, n0 + plane + en};
+    const int par = s->cur;
+    const double dt = s->deltatime;
+    int i, j, k;
+    elem_ijk(f, el, i, j, k);
+    const int en = f.nx + 1;
+    const int n0 = (k * en + j) * en + i;
+    const int pl = en * en;
+    const int nodes[8] = {n0,      n0 + 1,      n0 + en + 1,      n0 + en,
+                          n0 + pl, n0 + pl + 1, n0 + pl + en + 1, n0 + pl + en};
+    const double *__restrict__ px = f.x[par];
+    const double *__restrict__ py = f.y[par];
+    const double *__restrict__ pz = f.z[par];
+    const double *__restrict__ pxd = f.xd[par];
+    const double *__restrict__ pyd = f.yd[par];
+    const double *__restrict__ pzd = f.zd[par];
     double x[8], y[8], z[8], xv[8], yv[8], zv[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-        x[c] = f.x[nodes[c]];
-        y[c] = f.y[nodes[c]];
-        z[c] = f.z[nodes[c]];
-        xv[c] = f.xd[nodes[c]];
-        yv[c] = f.yd[nodes[c]];
-        zv[c] = f.zd[nodes[c]];
+        x[c] = px[nodes[c]];
+        y[c] = py[nodes[c]];
+        z[c] = pz[nodes[c]];
+        xv[c] = pxd[nodes[c]];
+        yv[c] = pyd[nodes[c]];
+        zv[c] = pzd[nodes[c]];
     }
     const double volo = f.volo[el];
-    const double dt = s->deltatime;
 
     // kinematics
     const double volume = elem_volume(x, y, z);
@@ -437,6 +281,29 @@ __device__ __forceinline__ double mono_phi(double self, double delvm,
     return phi;
 }
 
+// minimum over the block of a value given as the bit pattern of a
+// non-negative double (ordered like the values), returned to thread 0
+__device__ __forceinline__ unsigned long long block_min(unsigned long long v,
+                                                        unsigned long long *lds)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off, 64);
+        v = (o < v) ? o : v;
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        lds[wave] = v;
+    }
+    __syncthreads();
+    unsigned long long m = lds[0];
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int w = 1; w < nw; ++w) {
+        m = (lds[w] < m) ? lds[w] : m;
+    }
+    return m;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -446,6 +313,9 @@ __device__ __forceinline__ double mono_phi(double self, double delvm,
 // adjacent element of the structured cube; x = 0, y = 0 and z = 0 faces are
 // symmetry planes (neighbour value = own value), the far faces free surfaces
 // (neighbour value = 0) (reference/C/LULESH-OMP.cc:3153-3171, :2204-2287).
+// The Courant and hydro minima are exact and order independent: each block
+// folds its minimum into the cycle's accumulator with an unsigned 64-bit
+// atomic min on the bit pattern, and the last block to finish applies them.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_q_eos_dt(DevFields f, DevScalars *s,
                                                   PhysParams prm)
@@ -456,29 +326,27 @@ __global__ __launch_bounds__(256) void k_q_eos_dt(DevFields f, DevScalars *s,
     // once per block so that every thread takes the same path to the
     // barriers of the reduction
     if (threadIdx.x == 0) {
-        skip = (!cycle_active(s) || s->error != 0) ? 1 : 0;
+        skip = (s->active == 0 || s->error != 0) ? 1 : 0;
     }
     __syncthreads();
     if (skip) {
         return;
     }
-    const int64_t slot = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int slot = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
     unsigned long long cand_c = as_bits(kBig), cand_h = as_bits(kBig);
     if (slot < f.num_elem) {
-        const int el32 = f.eos_order[slot];
-        const int64_t el = el32;
+        const int el = f.eos_order[slot];
         const int rep = f.eos_rep[slot];
         const int nx = f.nx;
-        const int i = int(el % nx);
-        const int j = int((el / nx) % nx);
-        const int k = int(el / (int64_t(nx) * nx));
-        const int64_t sx = 1, sy = nx, sz = int64_t(nx) * nx;
+        int i, j, k;
+        elem_ijk(f, el, i, j, k);
+        const int sy = nx, sz = nx * nx;
 
         // monotonic Q limiter and ql / qq
         const double dvxi = f.delv_xi[el], dveta = f.delv_eta[el], dvzeta = f.delv_zeta[el];
         const double lm = prm.monoq_limiter_mult, ms = prm.monoq_max_slope;
-        const double phixi = mono_phi(dvxi, (i == 0) ? dvxi : f.delv_xi[el - sx],
-                                      (i == nx - 1) ? 0.0 : f.delv_xi[el + sx], lm, ms);
+        const double phixi = mono_phi(dvxi, (i == 0) ? dvxi : f.delv_xi[el - 1],
+                                      (i == nx - 1) ? 0.0 : f.delv_xi[el + 1], lm, ms);
         const double phieta = mono_phi(dveta, (j == 0) ? dveta : f.delv_eta[el - sy],
                                        (j == nx - 1) ? 0.0 : f.delv_eta[el + sy], lm, ms);
         const double phizeta = mono_phi(dvzeta, (k == 0) ? dvzeta : f.delv_zeta[el - sz],
@@ -585,28 +453,25 @@ __global__ __launch_bounds__(256) void k_q_eos_dt(DevFields f, DevScalars *s,
             cand_h = as_bits(dtdvov);
         }
     }
-    block_min_to(cand_c, &s->acc_courant, lds_c);
-    block_min_to(cand_h, &s->acc_hydro, lds_h);
+    // fold the block minima into the cycle's accumulators; they are applied
+    // to dtcourant / dthydro after the kernel (apply_constraints).  No fence
+    // is needed: the next kernel on the stream sees the atomics' results.
+    const unsigned long long mc = block_min(cand_c, lds_c);
+    const unsigned long long mh = block_min(cand_h, lds_h);
+    if (threadIdx.x == 0) {
+        if (mc < as_bits(kBig)) atomicMin(&s->acc_courant, mc);
+        if (mh < as_bits(kBig)) atomicMin(&s->acc_hydro, mh);
+    }
 }
 
-// ---------------------------------------------------------------------------
-// The minima replace dtcourant / dthydro when some element produced one
-// below 1e20; 2.0 starts every cycle from 1e20 (lulesh-2.0.3/lulesh.cc:
-// 2616-2620), 1.0 keeps the previous value otherwise (:2867-2871).
-// ---------------------------------------------------------------------------
+// applies the minima of the last cycle: 2.0 starts every cycle from 1e20
+// (lulesh-2.0.3/lulesh.cc:2616-2620); 1.0 keeps the previous constraint when
+// no element gave one (reference/C/LULESH-OMP.cc:2867-2871)
 __global__ void k_dt_finish(DevScalars *s, int ic_mode)
 {
-    if (threadIdx.x != 0 || !cycle_active(s) || s->error != 0) {
-        return;
+    if (threadIdx.x == 0) {
+        apply_constraints(s, ic_mode);
     }
-    const double c = __longlong_as_double(static_cast<long long>(s->acc_courant));
-    const double h = __longlong_as_double(static_cast<long long>(s->acc_hydro));
-    if (ic_mode == 2) {
-        s->dtcourant = kBig;
-        s->dthydro = kBig;
-    }
-    if (c < kBig) s->dtcourant = c;
-    if (h < kBig) s->dthydro = h;
 }
 
 // device math probe: op 0 = lulesh_cbrt(a), 1 = sqrt(a), 2 = a / b

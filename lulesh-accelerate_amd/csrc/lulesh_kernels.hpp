@@ -6,6 +6,11 @@
 // Nodes are indexed (k * (nx+1) + j) * (nx+1) + i and elements
 // (k * nx + j) * nx + i, x fastest, as in the reference
 // (reference/C/LULESH-OMP.cc:2996-3042), so host buffers copy 1:1.
+//
+// Node positions and velocities are double-buffered: the force sweep reads
+// one copy and writes the other, so no block can see a neighbour's already
+// advanced nodes.  DevScalars::cur names the copy holding the current state;
+// TimeIncrement flips it when a cycle starts.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -30,8 +35,10 @@ struct DevScalars {
     unsigned long long acc_hydro;
     int cycle;
     int max_cycles;
-    int active;  // 1 while the current cycle runs
-    int error;   // 0, -1 VolumeError, -2 QStopError
+    int active;        // 1 while the current cycle runs
+    int error;         // 0, -1 VolumeError, -2 QStopError
+    int dt_pending;  // the accumulators hold minima not yet applied
+    int cur;         // copy of the node arrays holding the state
 };
 
 // Constants of the method (reference/C/LULESH-OMP.cc:3063-3087;
@@ -44,29 +51,65 @@ struct PhysParams {
     int ic_mode;
 };
 
+// n / d and n % d for 0 <= n < 2^31 by multiply-high (Granlund-Montgomery):
+// with l = ceil(log2 d) and m = floor(2^32 (2^l - d) / d) + 1,
+// n / d = (umulhi(n, m) + n) >> l.
+struct FastDiv {
+    unsigned int d;
+    unsigned int m;
+    unsigned int l;
+};
+
+inline FastDiv make_fastdiv(unsigned int d)
+{
+    unsigned int l = 0;
+    while ((1ull << l) < d) {
+        ++l;
+    }
+    const unsigned long long m =
+        ((1ull << 32) * ((1ull << l) - d)) / d + 1;
+    return FastDiv{d, static_cast<unsigned int>(m), l};
+}
+
+__device__ __forceinline__ unsigned int fast_div(unsigned int n, const FastDiv &f)
+{
+    return (__umulhi(n, f.m) + n) >> f.l;
+}
+
 struct DevFields {
     int nx;
-    int64_t num_elem, num_node;
-    // node-centred
-    double *x, *y, *z, *xd, *yd, *zd, *nodal_mass;
+    int num_elem, num_node;
+    FastDiv div_nx;
+    // node-centred: positions and velocities in two copies (see above)
+    double *x[2], *y[2], *z[2], *xd[2], *yd[2], *zd[2];
+    double *nodal_mass;
     // element-centred
     double *e, *p, *q, *ql, *qq, *v, *volo, *elem_mass, *ss;
     double *vnew, *delv, *arealg, *vdov;
     double *delv_xi, *delv_eta, *delv_zeta, *delx_xi, *delx_eta, *delx_zeta;
-    // corner forces: 6 components x 8 corners x num_elem
-    double *corner;
     // region-ordered EOS schedule
     const int32_t *eos_order;
     const uint16_t *eos_rep;
 };
 
-__global__ void k_time_increment(DevScalars *s);
-__global__ void k_corner_forces(DevFields f, DevScalars *s, PhysParams prm);
-__global__ void k_nodal_update(DevFields f, DevScalars *s, PhysParams prm);
+// Launch shape of the force sweep: TX x TY element columns per block (one
+// thread each), NZ node planes per block, and the number of waves per SIMD
+// the kernel's register budget must leave room for.
+struct SweepShape {
+    int tx, ty, nz, waves;
+};
+
+__global__ void k_time_increment(DevScalars *s, int ic_mode);
 __global__ void k_kinematics_q(DevFields f, DevScalars *s, PhysParams prm);
 __global__ void k_q_eos_dt(DevFields f, DevScalars *s, PhysParams prm);
 __global__ void k_dt_finish(DevScalars *s, int ic_mode);
 __global__ void k_device_math(int op, int n, const double *a, const double *b,
                               double *out);
+
+// Launches the fused force / nodal-update sweep with the given shape.
+hipError_t launch_force_sweep(const SweepShape &shape, const DevFields &f,
+                              DevScalars *s, const PhysParams &prm,
+                              hipStream_t stream);
+bool force_sweep_shape_supported(const SweepShape &shape);
 
 }  // namespace lulesh

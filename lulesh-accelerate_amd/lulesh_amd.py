@@ -57,7 +57,8 @@ EXPORTS = (
     "lulesh_config_default", "lulesh_create", "lulesh_destroy", "lulesh_step",
     "lulesh_run", "lulesh_upload_state", "lulesh_download_state",
     "lulesh_get_field", "lulesh_set_field", "lulesh_get_stats",
-    "lulesh_set_scalars", "lulesh_get_regions", "lulesh_symmetry",
+    "lulesh_set_scalars", "lulesh_set_parameter", "lulesh_set_sweep_shape",
+    "lulesh_get_sweep_shape", "lulesh_get_regions", "lulesh_symmetry",
     "lulesh_kernel_count", "lulesh_kernel_name", "lulesh_set_kernel_timing",
     "lulesh_kernel_times", "lulesh_last_step_ms", "lulesh_last_error",
     "lulesh_device_math",
@@ -99,6 +100,12 @@ def lib() -> ctypes.CDLL:
     L.lulesh_get_stats.restype = I
     L.lulesh_set_scalars.argtypes = [P, DP, DP, DP, DP, IP]
     L.lulesh_set_scalars.restype = I
+    L.lulesh_set_parameter.argtypes = [P, ctypes.c_char_p, D]
+    L.lulesh_set_parameter.restype = I
+    L.lulesh_set_sweep_shape.argtypes = [P, I, I, I, I]
+    L.lulesh_set_sweep_shape.restype = I
+    L.lulesh_get_sweep_shape.argtypes = [P, IP, IP, IP, IP]
+    L.lulesh_get_sweep_shape.restype = I
     L.lulesh_get_regions.argtypes = [P, IP]
     L.lulesh_get_regions.restype = I
     L.lulesh_symmetry.argtypes = [P, DP, DP, DP]
@@ -220,6 +227,19 @@ class Domain:
         c = ctypes.byref(ctypes.c_int(int(cycle))) if cycle is not None else None
         self._check(lib().lulesh_set_scalars(self.h, dref(time), dref(deltatime),
                                              dref(dtcourant), dref(dthydro), c))
+
+    def set_parameter(self, name: str, value: float) -> None:
+        """Sets a constant of the method (hgcoef, qstop, ..., stoptime)."""
+        self._check(lib().lulesh_set_parameter(self.h, name.encode(),
+                                               float(value)))
+
+    def set_sweep_shape(self, tx: int, ty: int, nz: int, waves: int) -> None:
+        self._check(lib().lulesh_set_sweep_shape(self.h, tx, ty, nz, waves))
+
+    def sweep_shape(self) -> tuple[int, int, int, int]:
+        v = [ctypes.c_int() for _ in range(4)]
+        self._check(lib().lulesh_get_sweep_shape(self.h, *[ctypes.byref(a) for a in v]))
+        return tuple(a.value for a in v)
 
     def regions(self) -> np.ndarray:
         out = np.empty(self.num_elem, np.int32)

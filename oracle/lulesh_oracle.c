@@ -1542,12 +1542,29 @@ double lo_scalar(const lo_domain *d, const char *name)
 
 void lo_set_scalar(lo_domain *d, const char *name, double value)
 {
-    if (strcmp(name, "time") == 0) d->time = value;
-    else if (strcmp(name, "deltatime") == 0) d->deltatime = value;
-    else if (strcmp(name, "dtcourant") == 0) d->dtcourant = value;
-    else if (strcmp(name, "dthydro") == 0) d->dthydro = value;
-    else if (strcmp(name, "stoptime") == 0) d->stoptime = value;
-    else if (strcmp(name, "cycle") == 0) d->cycle = (int)value;
+    struct { const char *name; double *p; } t[] = {
+        {"time", &d->time}, {"deltatime", &d->deltatime},
+        {"dtcourant", &d->dtcourant}, {"dthydro", &d->dthydro},
+        {"stoptime", &d->stoptime}, {"dtmax", &d->dtmax},
+        {"dtfixed", &d->dtfixed}, {"hgcoef", &d->hgcoef},
+        {"qstop", &d->qstop}, {"monoq_max_slope", &d->monoq_max_slope},
+        {"monoq_limiter_mult", &d->monoq_limiter_mult},
+        {"qlc_monoq", &d->qlc_monoq}, {"qqc_monoq", &d->qqc_monoq},
+        {"qqc", &d->qqc}, {"eosvmax", &d->eosvmax}, {"eosvmin", &d->eosvmin},
+        {"pmin", &d->pmin}, {"emin", &d->emin}, {"dvovmax", &d->dvovmax},
+        {"refdens", &d->refdens}, {"e_cut", &d->e_cut}, {"p_cut", &d->p_cut},
+        {"q_cut", &d->q_cut}, {"u_cut", &d->u_cut}, {"v_cut", &d->v_cut},
+    };
+    if (strcmp(name, "cycle") == 0) {
+        d->cycle = (int)value;
+        return;
+    }
+    for (size_t i = 0; i < sizeof(t) / sizeof(t[0]); ++i) {
+        if (strcmp(t[i].name, name) == 0) {
+            *t[i].p = value;
+            return;
+        }
+    }
 }
 
 /* ------------------------------------------------------------------------ */

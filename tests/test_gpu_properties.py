@@ -60,18 +60,25 @@ def test_checkpoint_resume_is_exact(lulesh_gpu):
     b.close()
 
 
-def test_qstop_error(lulesh_gpu):
-    """q above qstop (1e12) stops the run with QStopError, as the reference
-    exits with QStopError (reference/C/LULESH-OMP.cc:2368-2381)."""
+def test_qstop_error(lulesh_gpu, oracle):
+    """q above qstop stops the run with QStopError, as the reference exits
+    with QStopError (reference/C/LULESH-OMP.cc:2368-2381); the check reads the
+    q of the previous cycle.  The oracle stops at the same cycle."""
     d = lulesh_gpu.Domain(8, ic_mode=2)
-    q = d.get("q")
-    q[17] = 2.0e12
-    d.set("q", q)
+    assert d.step(10) == (0, 10)
+    qmax = d.get("q").max()
+    assert qmax > 0
+    d.set_parameter("qstop", qmax / 2)
     rc, n = d.step(5, raise_on_error=False)
     assert rc == lulesh_gpu.QSTOP_ERROR
     assert n == 1
     # the domain stays in the error state
     assert d.step(1, raise_on_error=False)[0] == lulesh_gpu.QSTOP_ERROR
+    o = oracle.Oracle(8, mode=2)
+    assert o.run(10) == 0
+    o.set_scalar("qstop", qmax / 2)
+    assert o.run(15) == -2
+    assert o.cycle == d.stats().cycle == 11
     d.close()
 
 
@@ -88,13 +95,12 @@ def test_volume_error(lulesh_gpu):
 
 
 def test_tangled_mesh_volume_error(lulesh_gpu):
-    """Folding an element inside out makes its Jacobian volume negative."""
+    """Folding a layer of elements inside out makes their Jacobian volume
+    negative (reference/C/LULESH-OMP.cc:1579-1584)."""
     d = lulesh_gpu.Domain(6, ic_mode=2)
-    x = d.get("x")
-    en = 7
-    node = (2 * en + 2) * en + 2
-    x[node] += 0.5  # past its +x neighbour (spacing 0.1875)
-    d.set("x", x)
+    x = d.get("x").reshape(7, 7, 7)
+    x[:, :, 2] += 0.4  # the i = 2 node plane moves past i = 3 (spacing 0.1875)
+    d.set("x", x.ravel())
     rc, _ = d.step(1, raise_on_error=False)
     assert rc == lulesh_gpu.VOLUME_ERROR
     d.close()
