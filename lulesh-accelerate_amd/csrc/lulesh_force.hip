@@ -8,17 +8,18 @@
 //
 // The reference writes 8 corner forces per element to memory and gathers
 // them per node in ascending element order (:905-926, :1475-1496).  Here a
-// block owns a (TX-1) x (TY-1) column of nodes and NZ node planes.  Its TX x
+// block owns a (TX-1) x (TY-1) column of nodes and nz node planes.  Its TX x
 // TY threads each compute one element column (the owned node columns plus a
 // one-element halo on the low side), sweeping upward one element layer at a
-// time.  The corner forces of a layer are staged in LDS, one component at a
-// time, and each owned node folds them in the reference's order: the eight
-// elements around node (I,J,K) in ascending index are (I-1,J-1,K-1) corner 6,
-// (I,J-1,K-1) 7, (I-1,J,K-1) 5, (I,J,K-1) 4, (I-1,J-1,K) 2, (I,J-1,K) 3,
-// (I-1,J,K) 1, (I,J,K) 0.  The first four come from the layer below and are
-// carried in registers between layers, so each sum is formed exactly as the
-// reference forms it: stress first (f = sum), then hourglass (f += sum),
-// every sum started from 0.0.  The corner forces never reach HBM.
+// time.  The corner forces of a layer are staged in LDS, the three stress
+// components together and then the three hourglass components, and each
+// owned node folds them in the reference's order: the eight elements around
+// node (I,J,K) in ascending index are (I-1,J-1,K-1) corner 6, (I,J-1,K-1) 7,
+// (I-1,J,K-1) 5, (I,J,K-1) 4, (I-1,J-1,K) 2, (I,J-1,K) 3, (I-1,J,K) 1,
+// (I,J,K) 0.  The first four come from the layer below and are carried in
+// registers between layers, so each sum is formed exactly as the reference
+// forms it: stress first (f = sum), then hourglass (f += sum), every sum
+// started from 0.0.  The corner forces never reach HBM.
 //
 // Node planes are read from the previous copy of the node arrays through
 // LDS (one global read per node per block) and the advanced nodes are
@@ -33,22 +34,28 @@ namespace lulesh {
 
 namespace {
 
-template <int TX, int TY, int NZ, int W>
+// node-by-node stencils of the volume derivative (reference/C/LULESH-OMP.cc:
+// 1008-1048)
+constexpr int kStencil[8][6] = {{1, 2, 3, 4, 5, 7}, {2, 3, 0, 5, 6, 4},
+                                {3, 0, 1, 6, 7, 5}, {0, 1, 2, 7, 4, 6},
+                                {7, 6, 5, 0, 3, 1}, {4, 7, 6, 1, 0, 2},
+                                {5, 4, 7, 2, 1, 3}, {6, 5, 4, 3, 2, 0}};
+
+template <int TX, int TY, int W>
 __global__ __launch_bounds__(TX *TY, W) void k_force_sweep(DevFields f, DevScalars *s,
-                                                        PhysParams prm)
+                                                        PhysParams prm, int nz)
 {
     constexpr int NT = TX * TY;
     constexpr int PX = TX + 1;
     constexpr int PN = (TX + 1) * (TY + 1);
-    __shared__ double plane[2][6][PN];  // node planes k and k+1: x y z xd yd zd
-    __shared__ double corner[8][NT];    // one force component of a layer
+    __shared__ double plane[2][6][PN];   // node planes k and k+1: x y z xd yd zd
+    __shared__ double corner[3][8][NT];  // three force components of a layer
     __shared__ int s_go, s_par;
     __shared__ double s_dt;
 
     const int tid = threadIdx.x;
     if (tid == 0) {
-        // other kernels of this cycle may not have run yet, but the flags
-        // are only changed by earlier kernels: read them once per block
+        // the flags are only changed by earlier kernels: read them once
         s_go = (s->active != 0 && s->error == 0) ? 1 : 0;
         s_par = s->cur;
         s_dt = s->deltatime;
@@ -68,8 +75,8 @@ __global__ __launch_bounds__(TX *TY, W) void k_force_sweep(DevFields f, DevScala
     const int ty = tid / TX;
     const int i0 = int(blockIdx.x) * (TX - 1);
     const int j0 = int(blockIdx.y) * (TY - 1);
-    const int kc0 = int(blockIdx.z) * NZ;
-    const int kc1 = min(kc0 + NZ, en);  // owned node planes [kc0, kc1)
+    const int kc0 = int(blockIdx.z) * nz;
+    const int kc1 = min(kc0 + nz, en);  // owned node planes [kc0, kc1)
     // this thread's element column; the node column (ie, je) is owned when
     // tx, ty >= 1
     const int ie = i0 - 1 + tx;
@@ -83,56 +90,80 @@ __global__ __launch_bounds__(TX *TY, W) void k_force_sweep(DevFields f, DevScala
     const bool v_00 = owner && ie < nx && je < nx;
     const bool hourglass = prm.hgcoef > 0.0;
 
-    auto load_plane = [&](int kp, int slot) {
-        for (int idx = tid; idx < PN; idx += NT) {
-            const int lx = idx % PX;
-            const int ly = idx / PX;
-            const int gi = i0 - 1 + lx;
-            const int gj = j0 - 1 + ly;
-            if (gi >= 0 && gi <= nx && gj >= 0 && gj <= nx) {
-                const int n = (kp * en + gj) * en + gi;
+    // A node plane of the tile is (TX+1) x (TY+1) nodes; thread tid holds the
+    // slots tid + r * NT (r < kSlots) of it.  Planes are fetched into
+    // registers one layer ahead and written to LDS when they are needed, so
+    // the global loads overlap the element arithmetic of the layer before.
+    constexpr int kSlots = (PN + NT - 1) / NT;
+    int gnode[kSlots];  // global node index of each slot in plane 0, or -1
+#pragma unroll
+    for (int r = 0; r < kSlots; ++r) {
+        const int idx = tid + r * NT;
+        const int lx = idx % PX;
+        const int ly = idx / PX;
+        const int gi = i0 - 1 + lx;
+        const int gj = j0 - 1 + ly;
+        gnode[r] = (idx < PN && gi >= 0 && gi <= nx && gj >= 0 && gj <= nx)
+                       ? gj * en + gi
+                       : -1;
+    }
+    const int plane_stride = en * en;
+    double pre[kSlots][6];
+    auto fetch_plane = [&](int kp) {
+#pragma unroll
+        for (int r = 0; r < kSlots; ++r) {
+            if (gnode[r] >= 0) {
+                const int n = kp * plane_stride + gnode[r];
 #pragma unroll
                 for (int fld = 0; fld < 6; ++fld) {
-                    plane[slot][fld][idx] = in[fld][n];
+                    pre[r][fld] = in[fld][n];
                 }
             } else {
 #pragma unroll
                 for (int fld = 0; fld < 6; ++fld) {
-                    plane[slot][fld][idx] = 0.0;
+                    pre[r][fld] = 0.0;
+                }
+            }
+        }
+    };
+    auto store_plane = [&](int slot) {
+#pragma unroll
+        for (int r = 0; r < kSlots; ++r) {
+            const int idx = tid + r * NT;
+            if (idx < PN) {
+#pragma unroll
+                for (int fld = 0; fld < 6; ++fld) {
+                    plane[slot][fld][idx] = pre[r][fld];
                 }
             }
         }
     };
 
-    // stage one force component of the layer and fold it into the owned
-    // node: F = (sum carried from the layer below) + corners 2,3,1,0; the
-    // carried sum for the plane above restarts from corners 6,7,5,4
-    auto stage = [&](const double *val, bool have, bool finalize, bool carry,
-                     bool layer, double &F, double &P) {
-        if (have) {
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                corner[c][tid] = val[c];
-            }
-        }
+    // fold the three staged components into the owned node:
+    // F = (sum carried from the layer below) + corners 2,3,1,0; the carried
+    // sum for the plane above restarts from corners 6,7,5,4
+    auto fold = [&](bool finalize, bool carry, bool layer, double *F, double *P) {
         __syncthreads();
-        if (finalize) {
-            double acc = P;
-            if (layer) {
-                if (v_mm) acc += corner[2][tid - TX - 1];
-                if (v_0m) acc += corner[3][tid - TX];
-                if (v_m0) acc += corner[1][tid - 1];
-                if (v_00) acc += corner[0][tid];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            if (finalize) {
+                double acc = P[t];
+                if (layer) {
+                    if (v_mm) acc += corner[t][2][tid - TX - 1];
+                    if (v_0m) acc += corner[t][3][tid - TX];
+                    if (v_m0) acc += corner[t][1][tid - 1];
+                    if (v_00) acc += corner[t][0][tid];
+                }
+                F[t] = acc;
             }
-            F = acc;
-        }
-        if (carry) {
-            double acc = 0.0;
-            if (v_mm) acc += corner[6][tid - TX - 1];
-            if (v_0m) acc += corner[7][tid - TX];
-            if (v_m0) acc += corner[5][tid - 1];
-            if (v_00) acc += corner[4][tid];
-            P = acc;
+            if (carry) {
+                double acc = 0.0;
+                if (v_mm) acc += corner[t][6][tid - TX - 1];
+                if (v_0m) acc += corner[t][7][tid - TX];
+                if (v_m0) acc += corner[t][5][tid - 1];
+                if (v_00) acc += corner[t][4][tid];
+                P[t] = acc;
+            }
         }
         __syncthreads();
     };
@@ -142,14 +173,21 @@ __global__ __launch_bounds__(TX *TY, W) void k_force_sweep(DevFields f, DevScala
 
     double P[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const int kbeg = (kc0 > 0) ? kc0 - 1 : 0;
-    load_plane(kbeg, kbeg & 1);
+    fetch_plane(kbeg);
+    store_plane(kbeg & 1);
+    if (kbeg < nx) {
+        fetch_plane(kbeg + 1);
+    }
     for (int k = kbeg; k < kc1; ++k) {
-        __syncthreads();
         const bool layer = k < nx;
+        __syncthreads();  // the readers of the slot below are done
         if (layer) {
-            load_plane(k + 1, (k + 1) & 1);
+            store_plane((k + 1) & 1);  // plane k+1, fetched one layer ago
         }
         __syncthreads();
+        if (layer && k + 1 < nx && k + 1 < kc1) {
+            fetch_plane(k + 2);  // in flight while this layer computes
+        }
         const int lo = k & 1;
         const int hi = lo ^ 1;
         const bool finalize = owner && k >= kc0;
@@ -158,8 +196,8 @@ __global__ __launch_bounds__(TX *TY, W) void k_force_sweep(DevFields f, DevScala
         const int el = compute ? (k * nx + je) * nx + ie : 0;
 
         double F[6];
-        // element corner coordinates from the LDS planes; read again after
-        // each barrier rather than kept live across the staging
+        // element corner coordinates from the LDS planes, read where needed
+        // rather than kept live across the staging
         auto coords = [&](double *x, double *y, double *z) {
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
@@ -173,35 +211,27 @@ __global__ __launch_bounds__(TX *TY, W) void k_force_sweep(DevFields f, DevScala
         };
 
         // ---- stress: f = -sigma * node normal (:864-903) ----
-        {
-            double pf[3][8];
-            if (compute) {
-                double x[8], y[8], z[8];
-                coords(x, y, z);
-                const double sig = -f.p[el] - f.q[el];
-                const double det = jacobian_volume(x, y, z);
-                if (det <= 0.0) {
-                    atomicCAS(&s->error, 0, -1);
-                }
-                node_normals(pf[0], pf[1], pf[2], x, y, z);
-#pragma unroll
-                for (int t = 0; t < 3; ++t) {
-#pragma unroll
-                    for (int c = 0; c < 8; ++c) {
-                        pf[t][c] = -(sig * pf[t][c]);
-                    }
-                }
+        if (compute) {
+            double x[8], y[8], z[8], pf[3][8];
+            coords(x, y, z);
+            const double sig = -f.p[el] - f.q[el];
+            const double det = jacobian_volume(x, y, z);
+            if (det <= 0.0) {
+                atomicCAS(&s->error, 0, -1);
             }
+            node_normals(pf[0], pf[1], pf[2], x, y, z);
 #pragma unroll
             for (int t = 0; t < 3; ++t) {
-                stage(pf[t], compute, finalize, carry, layer, F[t], P[t]);
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    corner[t][c][tid] = -(sig * pf[t][c]);
+                }
             }
         }
+        fold(finalize, carry, layer, F, P);
 
         // ---- hourglass control (:1291-1473, :1516-1547) ----
         if (hourglass) {
-            double hourgam[8][4];
-            double coefficient = 0.0;
             if (compute) {
                 const double vv = f.v[el];
                 if (vv <= 0.0) {
@@ -209,51 +239,48 @@ __global__ __launch_bounds__(TX *TY, W) void k_force_sweep(DevFields f, DevScala
                 }
                 const double determ = f.volo[el] * vv;
                 const double volinv = 1.0 / determ;
-                double x[8], y[8], z[8];
-                coords(x, y, z);
-                // hourglass modes of the coordinates (:1308-1324)
-                double hm[4][3];
-#pragma unroll
-                for (int i1 = 0; i1 < 4; ++i1) {
-                    double hmx = x[0] * fb_gamma(i1, 0);
-                    double hmy = y[0] * fb_gamma(i1, 0);
-                    double hmz = z[0] * fb_gamma(i1, 0);
-#pragma unroll
-                    for (int n = 1; n < 8; ++n) {
-                        hmx = hmx + x[n] * fb_gamma(i1, n);
-                        hmy = hmy + y[n] * fb_gamma(i1, n);
-                        hmz = hmz + z[n] * fb_gamma(i1, n);
-                    }
-                    hm[i1][0] = hmx;
-                    hm[i1][1] = hmy;
-                    hm[i1][2] = hmz;
-                }
-                // node by node: its volume derivative (:1008-1048), then its
-                // row of hourgam (:1326-1356)
-                constexpr int kStencil[8][6] = {
-                    {1, 2, 3, 4, 5, 7}, {2, 3, 0, 5, 6, 4}, {3, 0, 1, 6, 7, 5},
-                    {0, 1, 2, 7, 4, 6}, {7, 6, 5, 0, 3, 1}, {4, 7, 6, 1, 0, 2},
-                    {5, 4, 7, 2, 1, 3}, {6, 5, 4, 3, 2, 0}};
-#pragma unroll
-                for (int n = 0; n < 8; ++n) {
-                    const int *st = kStencil[n];
-                    double dvx, dvy, dvz;
-                    volu_der(x, y, z, st[0], st[1], st[2], st[3], st[4], st[5], dvx, dvy,
-                             dvz);
+                double hourgam[8][4];
+                {
+                    double x[8], y[8], z[8];
+                    coords(x, y, z);
+                    // hourglass modes of the coordinates (:1308-1324)
+                    double hm[4][3];
 #pragma unroll
                     for (int i1 = 0; i1 < 4; ++i1) {
-                        hourgam[n][i1] =
-                            fb_gamma(i1, n) -
-                            volinv * (dvx * hm[i1][0] + dvy * hm[i1][1] + dvz * hm[i1][2]);
+                        double hmx = x[0] * fb_gamma(i1, 0);
+                        double hmy = y[0] * fb_gamma(i1, 0);
+                        double hmz = z[0] * fb_gamma(i1, 0);
+#pragma unroll
+                        for (int n = 1; n < 8; ++n) {
+                            hmx = hmx + x[n] * fb_gamma(i1, n);
+                            hmy = hmy + y[n] * fb_gamma(i1, n);
+                            hmz = hmz + z[n] * fb_gamma(i1, n);
+                        }
+                        hm[i1][0] = hmx;
+                        hm[i1][1] = hmy;
+                        hm[i1][2] = hmz;
+                    }
+                    // node by node: its volume derivative, then its row of
+                    // hourgam (:1326-1356)
+#pragma unroll
+                    for (int n = 0; n < 8; ++n) {
+                        const int *st = kStencil[n];
+                        double dvx, dvy, dvz;
+                        volu_der(x, y, z, st[0], st[1], st[2], st[3], st[4], st[5], dvx,
+                                 dvy, dvz);
+#pragma unroll
+                        for (int i1 = 0; i1 < 4; ++i1) {
+                            hourgam[n][i1] =
+                                fb_gamma(i1, n) -
+                                volinv * (dvx * hm[i1][0] + dvy * hm[i1][1] + dvz * hm[i1][2]);
+                        }
                     }
                 }
                 const double volume13 = lulesh_cbrt(determ);
-                coefficient = -prm.hgcoef * 0.01 * f.ss[el] * f.elem_mass[el] / volume13;
-            }
+                const double coefficient =
+                    -prm.hgcoef * 0.01 * f.ss[el] * f.elem_mass[el] / volume13;
 #pragma unroll
-            for (int t = 0; t < 3; ++t) {
-                double hgf[8];
-                if (compute) {
+                for (int t = 0; t < 3; ++t) {
                     double u[8];
 #pragma unroll
                     for (int c = 0; c < 4; ++c) {
@@ -270,12 +297,13 @@ __global__ __launch_bounds__(TX *TY, W) void k_force_sweep(DevFields f, DevScala
                     }
 #pragma unroll
                     for (int n = 0; n < 8; ++n) {
-                        hgf[n] = coefficient * (hourgam[n][0] * h[0] + hourgam[n][1] * h[1] +
-                                                hourgam[n][2] * h[2] + hourgam[n][3] * h[3]);
+                        corner[t][n][tid] =
+                            coefficient * (hourgam[n][0] * h[0] + hourgam[n][1] * h[1] +
+                                           hourgam[n][2] * h[2] + hourgam[n][3] * h[3]);
                     }
                 }
-                stage(hgf, compute, finalize, carry, layer, F[3 + t], P[3 + t]);
             }
+            fold(finalize, carry, layer, F + 3, P + 3);
         }
 
         // ---- acceleration, symmetry planes, velocity, position (:1613-1683)
@@ -310,37 +338,37 @@ __global__ __launch_bounds__(TX *TY, W) void k_force_sweep(DevFields f, DevScala
     }
 }
 
-template <int TX, int TY, int NZ, int W>
-hipError_t launch_shape(const DevFields &f, DevScalars *s, const PhysParams &prm,
-                        hipStream_t stream)
+template <int TX, int TY, int W>
+hipError_t launch_shape(int nz, const DevFields &f, DevScalars *s,
+                        const PhysParams &prm, hipStream_t stream)
 {
     const int en = f.nx + 1;
     const dim3 grid((en + TX - 2) / (TX - 1), (en + TY - 2) / (TY - 1),
-                    (en + NZ - 1) / NZ);
-    hipLaunchKernelGGL((k_force_sweep<TX, TY, NZ, W>), grid, dim3(TX * TY), 0, stream,
-                       f, s, prm);
+                    (en + nz - 1) / nz);
+    hipLaunchKernelGGL((k_force_sweep<TX, TY, W>), grid, dim3(TX * TY), 0, stream, f,
+                       s, prm, nz);
     return hipGetLastError();
 }
 
 }  // namespace
 
-// (TX, TY, NZ, minimum waves per SIMD the register budget must allow)
+// (TX, TY, minimum waves per SIMD the register budget must allow); the
+// number of node planes per block is a launch argument
 #define LULESH_SWEEP_SHAPES(X) \
-    X(32, 8, 8, 1)             \
-    X(32, 8, 16, 1)            \
-    X(32, 8, 32, 1)            \
-    X(32, 8, 16, 2)            \
-    X(16, 16, 16, 1)           \
-    X(16, 16, 16, 2)           \
-    X(16, 8, 16, 1)            \
-    X(16, 8, 16, 2)            \
-    X(64, 4, 16, 1)            \
-    X(32, 4, 16, 2)
+    X(32, 8, 1)                \
+    X(32, 8, 2)                \
+    X(16, 16, 2)               \
+    X(16, 8, 2)                \
+    X(32, 4, 2)                \
+    X(64, 4, 1)
 
 bool force_sweep_shape_supported(const SweepShape &sh)
 {
-#define LULESH_MATCH(a, b, c, w) \
-    if (sh.tx == a && sh.ty == b && sh.nz == c && sh.waves == w) return true;
+    if (sh.nz < 1 || sh.nz > 4096) {
+        return false;
+    }
+#define LULESH_MATCH(a, b, w) \
+    if (sh.tx == a && sh.ty == b && sh.waves == w) return true;
     LULESH_SWEEP_SHAPES(LULESH_MATCH)
 #undef LULESH_MATCH
     return false;
@@ -350,9 +378,12 @@ hipError_t launch_force_sweep(const SweepShape &sh, const DevFields &f,
                               DevScalars *s, const PhysParams &prm,
                               hipStream_t stream)
 {
-#define LULESH_LAUNCH(a, b, c, w)                                      \
-    if (sh.tx == a && sh.ty == b && sh.nz == c && sh.waves == w)         \
-        return launch_shape<a, b, c, w>(f, s, prm, stream);
+    if (sh.nz < 1) {
+        return hipErrorInvalidValue;
+    }
+#define LULESH_LAUNCH(a, b, w)                            \
+    if (sh.tx == a && sh.ty == b && sh.waves == w)        \
+        return launch_shape<a, b, w>(sh.nz, f, s, prm, stream);
     LULESH_SWEEP_SHAPES(LULESH_LAUNCH)
 #undef LULESH_LAUNCH
     return hipErrorInvalidValue;
