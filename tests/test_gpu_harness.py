@@ -1,0 +1,42 @@
+"""The C++ harness links the C ABI and prints the reference's verification
+block; its numbers match the published LULESH 2.0 answer at 30^3."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HARNESS = os.path.join(ROOT, "lulesh-accelerate_amd", "lulesh_mi355x")
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+
+def run(args):
+    out = subprocess.run([HARNESS] + args, capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stderr
+    return out.stdout
+
+
+def test_harness_30_cubed_2_0():
+    import lulesh_oracle
+    res = lulesh_oracle.parse_reference_output(run(["-s", "30"]))
+    assert res["cycles"] == 932
+    assert f"{res['origin_energy']:12.6e}" == f"{2.025075e+05:12.6e}"
+    assert abs(res["max_rel_diff"] - 1.461e-12) <= 1e-12
+
+
+def test_harness_1_0_and_cycle_limit():
+    import lulesh_oracle
+    res = lulesh_oracle.parse_reference_output(run(["-s", "10", "--ic", "1"]))
+    assert res["cycles"] == 593
+    assert f"{res['origin_energy']:12.6e}" == f"{5.174622e+05:12.6e}"
+    res = lulesh_oracle.parse_reference_output(run(["-s", "20", "-i", "25"]))
+    assert res["cycles"] == 25
+
+
+def test_harness_rejects_bad_options():
+    out = subprocess.run([HARNESS, "-s", "abc"], capture_output=True, text=True)
+    assert out.returncode != 0
