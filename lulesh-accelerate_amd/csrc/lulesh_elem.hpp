@@ -312,6 +312,7 @@ __device__ __forceinline__ double characteristic_length(const double *x,
 struct EosConst {
     double e_cut, p_cut, q_cut, eosvmax, eosvmin, pmin, emin, rho0;
     double ss_tiny, ss_floor;
+    bool unit_rho0;  // rho0 == 1: x / rho0 is x exactly, the divide is skipped
 };
 
 struct EosOut {
@@ -320,9 +321,13 @@ struct EosOut {
 
 __device__ __forceinline__ double eos_sound(double pbvc, double e, double vv,
                                             double bvc, double p, double rho0,
-                                            double tiny, double floor_v)
+                                            bool unit_rho0, double tiny,
+                                            double floor_v)
 {
-    double s = (pbvc * e + vv * bvc * p) / rho0;
+    double s = (pbvc * e + vv * bvc * p);
+    if (!unit_rho0) {
+        s = s / rho0;
+    }
     if (s <= tiny) {
         s = floor_v;
     } else {
@@ -374,7 +379,7 @@ __device__ __forceinline__ EosOut eos_element(double vnewc, double e_old,
         q_new = 0.;
     } else {
         const double ssc = eos_sound(pbvc, e_new, vhalf * vhalf, bvc, p_half_step,
-                                     c.rho0, c.ss_tiny, c.ss_floor);
+                                     c.rho0, c.unit_rho0, c.ss_tiny, c.ss_floor);
         q_new = (ssc * ql_old + qq_old);
     }
     e_new = e_new + 0.5 * delvc * (3.0 * (p_old + q_old) - 4.0 * (p_half_step + q_new));
@@ -392,7 +397,7 @@ __device__ __forceinline__ EosOut eos_element(double vnewc, double e_old,
             q_tilde = 0.;
         } else {
             const double ssc = eos_sound(pbvc, e_new, vnewc * vnewc, bvc, p_new,
-                                         c.rho0, c.ss_tiny, c.ss_floor);
+                                         c.rho0, c.unit_rho0, c.ss_tiny, c.ss_floor);
             q_tilde = (ssc * ql_old + qq_old);
         }
         e_new = e_new - (7.0 * (p_old + q_old) - 8.0 * (p_half_step + q_new) +
@@ -405,13 +410,13 @@ __device__ __forceinline__ EosOut eos_element(double vnewc, double e_old,
     p_new = eos_pressure(bvc, e_new, vnewc, c);
     if (delvc <= 0.) {
         const double ssc = eos_sound(pbvc, e_new, vnewc * vnewc, bvc, p_new,
-                                     c.rho0, c.ss_tiny, c.ss_floor);
+                                     c.rho0, c.unit_rho0, c.ss_tiny, c.ss_floor);
         q_new = (ssc * ql_old + qq_old);
         if (fabs(q_new) < c.q_cut) q_new = 0.;
     }
     EosOut o;
-    o.ss = eos_sound(pbvc, e_new, vnewc * vnewc, bvc, p_new, c.rho0, c.ss_tiny,
-                     c.ss_floor);
+    o.ss = eos_sound(pbvc, e_new, vnewc * vnewc, bvc, p_new, c.rho0, c.unit_rho0,
+                     c.ss_tiny, c.ss_floor);
     o.p = p_new;
     o.e = e_new;
     o.q = q_new;

@@ -357,6 +357,7 @@ hipError_t launch_shape(int nz, const DevFields &f, DevScalars *s,
 #define LULESH_SWEEP_SHAPES(X) \
     X(32, 8, 1)                \
     X(32, 8, 2)                \
+    X(32, 16, 2)               \
     X(16, 16, 2)               \
     X(16, 8, 2)                \
     X(32, 4, 2)                \

@@ -409,6 +409,7 @@ __global__ __launch_bounds__(256) void k_q_eos_dt(DevFields f, DevScalars *s,
         ec.pmin = prm.pmin;
         ec.emin = prm.emin;
         ec.rho0 = prm.refdens;
+        ec.unit_rho0 = prm.refdens == 1.0;
         ec.ss_tiny = prm.ss_tiny;
         ec.ss_floor = prm.ss_floor;
         const double e_old = f.e[el], delvc = f.delv[el], p_old = f.p[el];
