@@ -167,6 +167,74 @@ const char *lulesh_last_error(const lulesh_domain *d);
 int lulesh_device_math(int op, int n, const double *a, const double *b,
                        double *out);
 
+/* ------------------------------------------------------------------------
+ * Ensembles.  Many independent Sedov domains advanced together on one GPU:
+ * each phase of a cycle is one persistent kernel over the tiles / element
+ * chunks of all members, every member keeps its own time, dt and stop test
+ * (the loop of reference/C/lulesh-2.0.3/lulesh.cc:2771-2781 run per
+ * member), and the cycle is replayed from a captured HIP graph.  The
+ * reference runs one domain per process; a caller replaces N runs of
+ * `lulesh` (src/Main.hs:90-109) by one batch.  Each member's results are
+ * bit-identical to a lulesh_create/lulesh_run of the same configuration.
+ * ------------------------------------------------------------------------ */
+typedef struct lulesh_batch lulesh_batch;
+
+/* An ensemble drawn from a seed: edge sizes uniform in [nx_min, nx_max] and
+ * origin-energy factors uniform in [efactor_min, efactor_max], both from
+ * the project's splitmix64 stream (nx first, then the factor, per member). */
+typedef struct {
+    int num_members;
+    int nx_min, nx_max;
+    double efactor_min, efactor_max;
+    unsigned long long seed;
+} lulesh_ensemble;
+
+/* 64 members, nx in [16, 64], factors in [0.8, 1.2], seed 0. */
+void lulesh_ensemble_default(lulesh_ensemble *e);
+/* Writes the num_members edge sizes and energy factors of the ensemble. */
+int lulesh_ensemble_draw(const lulesh_ensemble *e, int *nx, double *efactor);
+
+/* n members with configurations cfg[0..n-1] (all with the same ic_mode)
+ * and origin-energy factors efactor[0..n-1] (NULL: all 1.0); the factor
+ * multiplies the member's initial e(0), and in 2.0 mode its dt0 follows. */
+int lulesh_batch_create(const lulesh_config *cfg, const double *efactor, int n,
+                        lulesh_batch **out);
+void lulesh_batch_destroy(lulesh_batch *b);
+
+/* Advances every member that has cycles left by up to ncycles cycles;
+ * *pending (may be NULL) receives the number of members with cycles left
+ * afterwards.  A member that fails stops; the others go on. */
+int lulesh_batch_step(lulesh_batch *b, int ncycles, int *pending);
+/* Steps until no member has cycles left.  Returns LULESH_OK, or the error
+ * code of the first failed member. */
+int lulesh_batch_run(lulesh_batch *b);
+
+/* A member's scalars; the return value is its error code (LULESH_OK,
+ * LULESH_VOLUME_ERROR, LULESH_QSTOP_ERROR) or LULESH_BAD_ARG. */
+int lulesh_batch_get_stats(lulesh_batch *b, int member, lulesh_stats *out);
+int lulesh_batch_get_field(lulesh_batch *b, int member, int field,
+                           double *host_out);
+
+/* Launch shapes: the sweep shape (as lulesh_set_sweep_shape), the number of
+ * persistent blocks of the sweep and of the element kernels (0: one block
+ * per tile / chunk), and the cycles replayed per captured graph (0: no
+ * graph, plain launches).  Results do not depend on any of them. */
+int lulesh_batch_set_shape(lulesh_batch *b, int tx, int ty, int nz, int waves,
+                           int sweep_blocks, int elem_blocks,
+                           int cycles_per_graph);
+int lulesh_batch_get_shape(lulesh_batch *b, int *tx, int *ty, int *nz,
+                           int *waves, int *sweep_blocks, int *elem_blocks,
+                           int *cycles_per_graph);
+
+/* The autotuner: times trial_cycles cycles of each candidate launch shape
+ * from a saved copy of the state, restores the state and keeps the fastest
+ * shape.  *best_ms (may be NULL) receives its milliseconds per cycle. */
+int lulesh_batch_autotune(lulesh_batch *b, int trial_cycles, double *best_ms);
+
+/* Milliseconds between the HIP events around the last step/run call. */
+double lulesh_batch_last_ms(lulesh_batch *b);
+const char *lulesh_batch_last_error(const lulesh_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,9 +22,10 @@ LIB = os.path.join(PKG, "liblulesh_mi355x.so")
 HARNESS = os.path.join(PKG, "lulesh_mi355x")
 
 HIP_SOURCES = ["lulesh_kernels.hip", "lulesh_force.hip", "lulesh_abi.hip",
-               "lulesh_mesh.cpp"]
+               "lulesh_batch.hip", "lulesh_setup.cpp", "lulesh_mesh.cpp"]
 HEADERS = ["lulesh_kernels.hpp", "lulesh_elem.hpp", "lulesh_cbrt.h",
-           "lulesh_mesh.hpp"]
+           "lulesh_mesh.hpp", "lulesh_setup.hpp"]
+OBJ_DIR = os.path.join(PKG, "_build")
 
 COMMON_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
@@ -49,9 +50,23 @@ def _run(cmd: list[str]) -> None:
 
 
 def build_library() -> str:
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    """Compiles every source to an object (in parallel), then links."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    cc = hipcc()
+    jobs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
+        lang = ["-x", "hip"] if src.endswith(".hip") else []
+        jobs.append((obj, [cc] + COMMON_FLAGS + ["-fPIC"] + lang +
+                     ["-c", os.path.join(CSRC, src), "-o", obj]))
+    workers = max(1, min(len(jobs), os.cpu_count() or 1, 8))
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        list(pool.map(lambda j: _run(j[1]), jobs))
     tmp = LIB + ".tmp"
-    _run([hipcc()] + COMMON_FLAGS + ["-fPIC", "-shared", "-o", tmp] + srcs)
+    _run([cc, "--offload-arch=gfx950", "-fPIC", "-shared", "-o", tmp] +
+         [obj for obj, _ in jobs])
     os.replace(tmp, LIB)
     return LIB
 

@@ -16,6 +16,7 @@
 
 #include "lulesh_kernels.hpp"
 #include "lulesh_mesh.hpp"
+#include "lulesh_setup.hpp"
 
 using namespace lulesh;
 
@@ -42,8 +43,6 @@ constexpr SweepShape kDefaultSweep = {32, 8, 16, 2};
 
 constexpr int kBlock = 256;
 constexpr int kBatch = 32;  // cycles queued per host round trip in run()
-
-size_t round_up(size_t n, size_t m) { return (n + m - 1) / m * m; }
 
 struct TimedLaunch {
     int kernel;
@@ -193,40 +192,12 @@ struct FieldRef {
     int64_t n;
 };
 
-// par: the copy of the node arrays that holds the current state (cycle & 1)
+// par: the copy of the node arrays that holds the current state
 FieldRef field_ref(lulesh_domain *d, int id, int par)
 {
-    const DevFields &f = d->f;
-    const int64_t ne = d->num_elem, nn = d->num_node;
-    switch (id) {
-    case LULESH_F_X: return {f.x[par], nn};
-    case LULESH_F_Y: return {f.y[par], nn};
-    case LULESH_F_Z: return {f.z[par], nn};
-    case LULESH_F_XD: return {f.xd[par], nn};
-    case LULESH_F_YD: return {f.yd[par], nn};
-    case LULESH_F_ZD: return {f.zd[par], nn};
-    case LULESH_F_NODAL_MASS: return {f.nodal_mass, nn};
-    case LULESH_F_E: return {f.e, ne};
-    case LULESH_F_P: return {f.p, ne};
-    case LULESH_F_Q: return {f.q, ne};
-    case LULESH_F_QL: return {f.ql, ne};
-    case LULESH_F_QQ: return {f.qq, ne};
-    case LULESH_F_V: return {f.v, ne};
-    case LULESH_F_VOLO: return {f.volo, ne};
-    case LULESH_F_DELV: return {f.delv, ne};
-    case LULESH_F_VDOV: return {f.vdov, ne};
-    case LULESH_F_AREALG: return {f.arealg, ne};
-    case LULESH_F_SS: return {f.ss, ne};
-    case LULESH_F_ELEM_MASS: return {f.elem_mass, ne};
-    case LULESH_F_VNEW: return {f.vnew, ne};
-    case LULESH_F_DELV_XI: return {f.delv_xi, ne};
-    case LULESH_F_DELV_ETA: return {f.delv_eta, ne};
-    case LULESH_F_DELV_ZETA: return {f.delv_zeta, ne};
-    case LULESH_F_DELX_XI: return {f.delx_xi, ne};
-    case LULESH_F_DELX_ETA: return {f.delx_eta, ne};
-    case LULESH_F_DELX_ZETA: return {f.delx_zeta, ne};
-    default: return {nullptr, 0};
-    }
+    int64_t n = 0;
+    double *p = field_pointer(d->f, id, par, &n);
+    return {p, n};
 }
 
 int copy_in(lulesh_domain *d, double *dst, const double *src, int64_t n)
@@ -245,112 +216,12 @@ int copy_out(lulesh_domain *d, double *dst, const double *src, int64_t n)
 
 int build_device_domain(lulesh_domain *d, const HostMesh &m)
 {
-    const size_t ne = size_t(m.num_elem), nn = size_t(m.num_node);
-    const size_t pe = round_up(ne, 64), pn = round_up(nn, 64);
-    // arena: 13 node arrays (two copies of x..zd, nodal mass), 19 element
-    // arrays, the EOS schedule, the scalars
-    const size_t n_node_arrays = 13, n_elem_arrays = 19;
-    size_t bytes = (n_node_arrays * pn + n_elem_arrays * pe) * sizeof(double);
-    const size_t off_sched = bytes;
-    bytes += round_up(pe * sizeof(int32_t), 256) + round_up(pe * sizeof(uint16_t), 256);
-    const size_t off_scalars = bytes;
-    bytes += round_up(sizeof(DevScalars), 256);
+    const size_t bytes = domain_arena_bytes(m);
     HIP_TRY(d, hipMalloc(&d->arena, bytes));
     HIP_TRY(d, hipMemsetAsync(d->arena, 0, bytes, d->stream));
-
-    char *base = static_cast<char *>(d->arena);
-    double *cur = reinterpret_cast<double *>(base);
-    auto take_node = [&]() { double *p = cur; cur += pn; return p; };
-    auto take_elem = [&]() { double *p = cur; cur += pe; return p; };
-    DevFields &f = d->f;
-    f.nx = m.nx;
-    f.num_elem = int(m.num_elem);
-    f.num_node = int(m.num_node);
-    f.div_nx = make_fastdiv(unsigned(m.nx));
-    for (int c = 0; c < 2; ++c) {
-        f.x[c] = take_node(); f.y[c] = take_node(); f.z[c] = take_node();
-        f.xd[c] = take_node(); f.yd[c] = take_node(); f.zd[c] = take_node();
-    }
-    f.nodal_mass = take_node();
-    f.e = take_elem(); f.p = take_elem(); f.q = take_elem();
-    f.ql = take_elem(); f.qq = take_elem(); f.v = take_elem();
-    f.volo = take_elem(); f.elem_mass = take_elem(); f.ss = take_elem();
-    f.vnew = take_elem(); f.delv = take_elem(); f.arealg = take_elem();
-    f.vdov = take_elem();
-    f.delv_xi = take_elem(); f.delv_eta = take_elem(); f.delv_zeta = take_elem();
-    f.delx_xi = take_elem(); f.delx_eta = take_elem(); f.delx_zeta = take_elem();
-    int32_t *order = reinterpret_cast<int32_t *>(base + off_sched);
-    uint16_t *rep = reinterpret_cast<uint16_t *>(base + off_sched +
-                                                 round_up(pe * sizeof(int32_t), 256));
-    f.eos_order = order;
-    f.eos_rep = rep;
-    d->s = reinterpret_cast<DevScalars *>(base + off_scalars);
-
-    int rc;
-    if ((rc = copy_in(d, f.x[0], m.x.data(), m.num_node))) return rc;
-    if ((rc = copy_in(d, f.y[0], m.y.data(), m.num_node))) return rc;
-    if ((rc = copy_in(d, f.z[0], m.z.data(), m.num_node))) return rc;
-    if ((rc = copy_in(d, f.nodal_mass, m.nodal_mass.data(), m.num_node))) return rc;
-    if ((rc = copy_in(d, f.e, m.e.data(), m.num_elem))) return rc;
-    if ((rc = copy_in(d, f.volo, m.volo.data(), m.num_elem))) return rc;
-    if ((rc = copy_in(d, f.elem_mass, m.volo.data(), m.num_elem))) return rc;
-    {
-        std::vector<double> ones(ne, 1.0);
-        if ((rc = copy_in(d, f.v, ones.data(), m.num_elem))) return rc;
-        HIP_TRY(d, hipMemcpyAsync(order, m.eos_order.data(), ne * sizeof(int32_t),
-                                  hipMemcpyHostToDevice, d->stream));
-        HIP_TRY(d, hipMemcpyAsync(rep, m.eos_rep.data(), ne * sizeof(uint16_t),
-                                  hipMemcpyHostToDevice, d->stream));
-        HIP_TRY(d, hipStreamSynchronize(d->stream));
-    }
-
-    DevScalars s{};
-    s.time = 0.0;
-    s.deltatime = m.deltatime0;
-    s.dtcourant = 1.0e+20;
-    s.dthydro = 1.0e+20;
-    s.stoptime = d->cfg.stoptime;
-    s.dtfixed = m.dtfixed;
-    s.dtmax = 1.0e-2;
-    s.deltatimemultlb = 1.1;
-    s.deltatimemultub = 1.2;
-    double big = 1.0e+20;
-    std::memcpy(&s.acc_courant, &big, sizeof(big));
-    std::memcpy(&s.acc_hydro, &big, sizeof(big));
-    s.cycle = 0;
-    s.max_cycles = d->cfg.max_cycles;
-    s.active = 0;
-    s.error = 0;
-    return write_scalars(d, s);
-}
-
-PhysParams make_params(int ic_mode)
-{
-    PhysParams p{};
-    p.u_cut = 1.0e-7;
-    p.hgcoef = 3.0;
-    p.qstop = 1.0e+12;
-    p.monoq_max_slope = 1.0;
-    p.monoq_limiter_mult = 2.0;
-    p.e_cut = 1.0e-7;
-    p.p_cut = 1.0e-7;
-    p.q_cut = 1.0e-7;
-    p.v_cut = 1.0e-10;
-    p.qlc_monoq = 0.5;
-    p.qqc_monoq = 2.0 / 3.0;
-    p.qqc = 2.0;
-    p.eosvmax = 1.0e+9;
-    p.eosvmin = 1.0e-9;
-    p.pmin = 0.;
-    p.emin = -1.0e+15;
-    p.dvovmax = 0.1;
-    p.refdens = 1.0;
-    // sound-speed floor: reference/C/LULESH-OMP.cc:2453-2454 (1.0),
-    // lulesh-2.0.3/lulesh.cc:2113-2114 (2.0)
-    p.ss_tiny = (ic_mode == 1) ? .111111e-36 : .1111111e-36;
-    p.ss_floor = (ic_mode == 1) ? .333333e-18 : .3333333e-18;
-    p.ic_mode = ic_mode;
-    return p;
+    carve_domain(static_cast<char *>(d->arena), m, d->f, &d->s);
+    HIP_TRY(d, upload_mesh(m, d->f, d->stream));
+    return write_scalars(d, initial_scalars(m, d->cfg.stoptime, d->cfg.max_cycles));
 }
 
 }  // namespace

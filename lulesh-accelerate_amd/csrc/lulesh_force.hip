@@ -41,9 +41,13 @@ constexpr int kStencil[8][6] = {{1, 2, 3, 4, 5, 7}, {2, 3, 0, 5, 6, 4},
                                 {7, 6, 5, 0, 3, 1}, {4, 7, 6, 1, 0, 2},
                                 {5, 4, 7, 2, 1, 3}, {6, 5, 4, 3, 2, 0}};
 
-template <int TX, int TY, int W>
-__global__ __launch_bounds__(TX *TY, W) void k_force_sweep(DevFields f, DevScalars *s,
-                                                        PhysParams prm, int nz)
+// One block's share of the sweep: node columns [bx*(TX-1), (bx+1)*(TX-1)) x
+// [by*(TY-1), (by+1)*(TY-1)) and node planes [bz*nz, (bz+1)*nz) of one
+// domain.
+template <int TX, int TY>
+__device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
+                                           const PhysParams &prm, int nz, int bx,
+                                           int by, int bz)
 {
     constexpr int NT = TX * TY;
     constexpr int PX = TX + 1;
@@ -73,9 +77,9 @@ __global__ __launch_bounds__(TX *TY, W) void k_force_sweep(DevFields f, DevScala
 
     const int tx = tid % TX;
     const int ty = tid / TX;
-    const int i0 = int(blockIdx.x) * (TX - 1);
-    const int j0 = int(blockIdx.y) * (TY - 1);
-    const int kc0 = int(blockIdx.z) * nz;
+    const int i0 = bx * (TX - 1);
+    const int j0 = by * (TY - 1);
+    const int kc0 = bz * nz;
     const int kc1 = min(kc0 + nz, en);  // owned node planes [kc0, kc1)
     // this thread's element column; the node column (ie, je) is owned when
     // tx, ty >= 1
@@ -339,6 +343,30 @@ __global__ __launch_bounds__(TX *TY, W) void k_force_sweep(DevFields f, DevScala
 }
 
 template <int TX, int TY, int W>
+__global__ __launch_bounds__(TX *TY, W) void k_force_sweep(DevFields f, DevScalars *s,
+                                                        PhysParams prm, int nz)
+{
+    sweep_tile<TX, TY>(f, s, prm, nz, int(blockIdx.x), int(blockIdx.y),
+                       int(blockIdx.z));
+}
+
+// The ensemble form: a persistent grid walks the tile list of all members;
+// a tile names (member, bx, by, bz).  Members whose run is over or failed
+// leave their tiles at once (sweep_tile reads their flags).
+template <int TX, int TY, int W>
+__global__ __launch_bounds__(TX *TY, W) void k_force_sweep_batch(
+    const BatchMember *members, const BatchWork *work, PhysParams prm, int nz)
+{
+    const int n = work->num_tiles;
+    for (int t = int(blockIdx.x); t < n; t += int(gridDim.x)) {
+        const int4 w = work->tiles[t];
+        __syncthreads();  // the LDS of the previous tile is no longer read
+        const BatchMember &m = members[w.x];
+        sweep_tile<TX, TY>(m.f, m.s, prm, nz, w.y, w.z, w.w);
+    }
+}
+
+template <int TX, int TY, int W>
 hipError_t launch_shape(int nz, const DevFields &f, DevScalars *s,
                         const PhysParams &prm, hipStream_t stream)
 {
@@ -347,6 +375,16 @@ hipError_t launch_shape(int nz, const DevFields &f, DevScalars *s,
                     (en + nz - 1) / nz);
     hipLaunchKernelGGL((k_force_sweep<TX, TY, W>), grid, dim3(TX * TY), 0, stream, f,
                        s, prm, nz);
+    return hipGetLastError();
+}
+
+template <int TX, int TY, int W>
+hipError_t launch_shape_batch(int nz, int blocks, const BatchMember *members,
+                              const BatchWork *work, const PhysParams &prm,
+                              hipStream_t stream)
+{
+    hipLaunchKernelGGL((k_force_sweep_batch<TX, TY, W>), dim3(blocks), dim3(TX * TY), 0,
+                       stream, members, work, prm, nz);
     return hipGetLastError();
 }
 
@@ -385,6 +423,21 @@ hipError_t launch_force_sweep(const SweepShape &sh, const DevFields &f,
 #define LULESH_LAUNCH(a, b, w)                            \
     if (sh.tx == a && sh.ty == b && sh.waves == w)        \
         return launch_shape<a, b, w>(sh.nz, f, s, prm, stream);
+    LULESH_SWEEP_SHAPES(LULESH_LAUNCH)
+#undef LULESH_LAUNCH
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_force_sweep_batch(const SweepShape &sh, int blocks,
+                                    const BatchMember *members, const BatchWork *work,
+                                    const PhysParams &prm, hipStream_t stream)
+{
+    if (sh.nz < 1 || blocks < 1) {
+        return hipErrorInvalidValue;
+    }
+#define LULESH_LAUNCH(a, b, w)                                                  \
+    if (sh.tx == a && sh.ty == b && sh.waves == w)                              \
+        return launch_shape_batch<a, b, w>(sh.nz, blocks, members, work, prm, stream);
     LULESH_SWEEP_SHAPES(LULESH_LAUNCH)
 #undef LULESH_LAUNCH
     return hipErrorInvalidValue;

@@ -86,11 +86,10 @@ __device__ __forceinline__ void elem_ijk(const DevFields &f, int el, int &i, int
 // 194-251).  Decides whether this cycle runs at all: the loop condition of
 // the drivers (time < stoptime, and cycle < its for 2.0) is evaluated here.
 // ---------------------------------------------------------------------------
-__global__ void k_time_increment(DevScalars *s, int ic_mode)
+namespace {
+
+__device__ __forceinline__ void time_increment(DevScalars *s, int ic_mode)
 {
-    if (threadIdx.x != 0) {
-        return;
-    }
     apply_constraints(s, ic_mode);
     if (s->error != 0 || !(s->time < s->stoptime) || !(s->cycle < s->max_cycles)) {
         s->active = 0;
@@ -135,22 +134,35 @@ __global__ void k_time_increment(DevScalars *s, int ic_mode)
     s->dt_pending = 1;
 }
 
+}  // namespace
+
+__global__ void k_time_increment(DevScalars *s, int ic_mode)
+{
+    if (threadIdx.x == 0) {
+        time_increment(s, ic_mode);
+    }
+}
+
+// one thread per member
+__global__ void k_time_increment_batch(const BatchMember *members,
+                                       const BatchWork *work, int ic_mode)
+{
+    const int n = work->num_members;
+    for (int m = int(threadIdx.x); m < n; m += int(blockDim.x)) {
+        time_increment(members[m].s, ic_mode);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Kinematics (CalcKinematicsForElems + CalcLagrangeElements,
 // reference/C/LULESH-OMP.cc:1927-2032) and the monotonic-Q gradients
 // (CalcMonotonicQGradientsForElems, :2034-2180) of each element, sharing one
 // read of its eight advanced nodes.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_kinematics_q(DevFields f, DevScalars *s,
-                                                      PhysParams prm)
+namespace {
+
+__device__ __forceinline__ void kinematics_q(const DevFields &f, DevScalars *s, int el)
 {
-    if (s->active == 0 || s->error != 0) {
-        return;
-    }
-    const int el = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
-    if (el >= f.num_elem) {
-        return;
-    }
     const int par = s->cur;
     const double dt = s->deltatime;
     int i, j, k;
@@ -259,6 +271,39 @@ __global__ __launch_bounds__(256) void k_kinematics_q(DevFields f, DevScalars *s
     f.delv_eta[el] = ax * dxv + ay * dyv + az * dzv;
 }
 
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_kinematics_q(DevFields f, DevScalars *s,
+                                                      PhysParams prm)
+{
+    if (s->active == 0 || s->error != 0) {
+        return;
+    }
+    const int el = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
+    if (el < f.num_elem) {
+        kinematics_q(f, s, el);
+    }
+}
+
+// persistent blocks over the element chunks of all members
+__global__ __launch_bounds__(256) void k_kinematics_q_batch(const BatchMember *members,
+                                                            const BatchWork *work,
+                                                            PhysParams prm)
+{
+    const int n = work->num_chunks;
+    for (int c = int(blockIdx.x); c < n; c += int(gridDim.x)) {
+        const int2 w = work->chunks[c];
+        const BatchMember &m = members[w.x];
+        if (m.s->active == 0 || m.s->error != 0) {
+            continue;
+        }
+        const int el = w.y + int(threadIdx.x);
+        if (el < m.f.num_elem) {
+            kinematics_q(m.f, m.s, el);
+        }
+    }
+}
+
 namespace {
 
 // limiter of one direction (reference/C/LULESH-OMP.cc:2201-2228)
@@ -317,22 +362,15 @@ __device__ __forceinline__ unsigned long long block_min(unsigned long long v,
 // folds its minimum into the cycle's accumulator with an unsigned 64-bit
 // atomic min on the bit pattern, and the last block to finish applies them.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_q_eos_dt(DevFields f, DevScalars *s,
-                                                  PhysParams prm)
+namespace {
+
+// one block of 256 elements in schedule order; every thread of the block
+// calls it (the minima are reduced over the block)
+__device__ __forceinline__ void q_eos_dt_block(const DevFields &f, DevScalars *s,
+                                               const PhysParams &prm, int slot,
+                                               unsigned long long *lds_c,
+                                               unsigned long long *lds_h)
 {
-    __shared__ unsigned long long lds_c[4], lds_h[4];
-    __shared__ int skip;
-    // other blocks may raise an error while this one runs: read the flags
-    // once per block so that every thread takes the same path to the
-    // barriers of the reduction
-    if (threadIdx.x == 0) {
-        skip = (s->active == 0 || s->error != 0) ? 1 : 0;
-    }
-    __syncthreads();
-    if (skip) {
-        return;
-    }
-    const int slot = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
     unsigned long long cand_c = as_bits(kBig), cand_h = as_bits(kBig);
     if (slot < f.num_elem) {
         const int el = f.eos_order[slot];
@@ -465,6 +503,53 @@ __global__ __launch_bounds__(256) void k_q_eos_dt(DevFields f, DevScalars *s,
     }
 }
 
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_q_eos_dt(DevFields f, DevScalars *s,
+                                                  PhysParams prm)
+{
+    __shared__ unsigned long long lds_c[4], lds_h[4];
+    __shared__ int skip;
+    // other blocks may raise an error while this one runs: read the flags
+    // once per block so that every thread takes the same path to the
+    // barriers of the reduction
+    if (threadIdx.x == 0) {
+        skip = (s->active == 0 || s->error != 0) ? 1 : 0;
+    }
+    __syncthreads();
+    if (skip) {
+        return;
+    }
+    // the schedule's costly regions come last; blocks take it from the end
+    // so the long blocks start first and the short ones fill the tail
+    const int chunk = int(gridDim.x) - 1 - int(blockIdx.x);
+    q_eos_dt_block(f, s, prm, chunk * int(blockDim.x) + int(threadIdx.x), lds_c, lds_h);
+}
+
+// persistent blocks over the element chunks of all members; a chunk lies
+// inside one member, so its minima go to that member's accumulators
+__global__ __launch_bounds__(256) void k_q_eos_dt_batch(const BatchMember *members,
+                                                        const BatchWork *work,
+                                                        PhysParams prm)
+{
+    __shared__ unsigned long long lds_c[4], lds_h[4];
+    __shared__ int skip;
+    const int n = work->num_chunks;
+    for (int c = int(blockIdx.x); c < n; c += int(gridDim.x)) {
+        const int2 w = work->chunks[c];
+        const BatchMember &m = members[w.x];
+        __syncthreads();  // the previous chunk's reads of the LDS are done
+        if (threadIdx.x == 0) {
+            skip = (m.s->active == 0 || m.s->error != 0) ? 1 : 0;
+        }
+        __syncthreads();
+        if (skip) {
+            continue;
+        }
+        q_eos_dt_block(m.f, m.s, prm, w.y + int(threadIdx.x), lds_c, lds_h);
+    }
+}
+
 // applies the minima of the last cycle: 2.0 starts every cycle from 1e20
 // (lulesh-2.0.3/lulesh.cc:2616-2620); 1.0 keeps the previous constraint when
 // no element gave one (reference/C/LULESH-OMP.cc:2867-2871)
@@ -472,6 +557,34 @@ __global__ void k_dt_finish(DevScalars *s, int ic_mode)
 {
     if (threadIdx.x == 0) {
         apply_constraints(s, ic_mode);
+    }
+}
+
+// the same for every member, and the count of members that have cycles
+// left (not failed, time < stoptime, cycle < max_cycles); one block
+__global__ void k_dt_finish_batch(const BatchMember *members, BatchWork *work,
+                                  int ic_mode)
+{
+    __shared__ int pending;
+    if (threadIdx.x == 0) {
+        pending = 0;
+    }
+    __syncthreads();
+    const int n = work->num_members;
+    int mine = 0;
+    for (int m = int(threadIdx.x); m < n; m += int(blockDim.x)) {
+        DevScalars *s = members[m].s;
+        apply_constraints(s, ic_mode);
+        if (s->error == 0 && s->time < s->stoptime && s->cycle < s->max_cycles) {
+            ++mine;
+        }
+    }
+    if (mine) {
+        atomicAdd(&pending, mine);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        work->num_pending = pending;
     }
 }
 

@@ -99,6 +99,27 @@ struct SweepShape {
     int tx, ty, nz, waves;
 };
 
+// One member of an ensemble: its fields and loop scalars.  Members share the
+// method constants (PhysParams) and IC mode; each keeps its own time, dt and
+// stop test.
+struct BatchMember {
+    DevFields f;
+    DevScalars *s;
+};
+
+// Work lists of the persistent ensemble kernels, in device memory.  The
+// host rewrites them (between cycles) when members finish, so the kernels
+// read the counts from here rather than from their launch arguments.
+constexpr int kChunk = 256;  // elements per chunk = threads per element block
+struct BatchWork {
+    int num_members;
+    int num_tiles;    // force-sweep tiles (member, bx, by, bz)
+    int num_chunks;   // element chunks (member, first element)
+    int num_pending;  // members with cycles left, set by k_dt_finish_batch
+    int4 *tiles;
+    int2 *chunks;
+};
+
 __global__ void k_time_increment(DevScalars *s, int ic_mode);
 __global__ void k_kinematics_q(DevFields f, DevScalars *s, PhysParams prm);
 __global__ void k_q_eos_dt(DevFields f, DevScalars *s, PhysParams prm);
@@ -106,10 +127,25 @@ __global__ void k_dt_finish(DevScalars *s, int ic_mode);
 __global__ void k_device_math(int op, int n, const double *a, const double *b,
                               double *out);
 
+// ensemble forms of the same phases
+__global__ void k_time_increment_batch(const BatchMember *members,
+                                       const BatchWork *work, int ic_mode);
+__global__ void k_kinematics_q_batch(const BatchMember *members,
+                                     const BatchWork *work, PhysParams prm);
+__global__ void k_q_eos_dt_batch(const BatchMember *members, const BatchWork *work,
+                                 PhysParams prm);
+__global__ void k_dt_finish_batch(const BatchMember *members, BatchWork *work,
+                                  int ic_mode);
+
 // Launches the fused force / nodal-update sweep with the given shape.
 hipError_t launch_force_sweep(const SweepShape &shape, const DevFields &f,
                               DevScalars *s, const PhysParams &prm,
                               hipStream_t stream);
+// The ensemble sweep: `blocks` persistent blocks over work->tiles, which
+// must have been cut for the same shape (tx, ty, nz).
+hipError_t launch_force_sweep_batch(const SweepShape &shape, int blocks,
+                                    const BatchMember *members, const BatchWork *work,
+                                    const PhysParams &prm, hipStream_t stream);
 bool force_sweep_shape_supported(const SweepShape &shape);
 
 }  // namespace lulesh

@@ -164,6 +164,9 @@ HostMesh build_mesh(const MeshConfig &cfg)
     if (cfg.ic_mode == 2 && (cfg.num_reg < 1 || cfg.num_reg > 4096 || cfg.cost < 0 || cfg.cost > 1000 || cfg.balance < 0 || cfg.balance > 16)) {
         throw std::invalid_argument("need 1 <= num_reg <= 4096, 0 <= cost <= 1000, 0 <= balance <= 16");
     }
+    if (!(cfg.efactor > 0.0) || !(cfg.efactor < 1.0e6)) {
+        throw std::invalid_argument("efactor must be in (0, 1e6)");
+    }
     HostMesh m;
     const int nx = cfg.nx, en = nx + 1;
     m.nx = nx;
@@ -225,14 +228,16 @@ HostMesh build_mesh(const MeshConfig &cfg)
     m.e.assign(ne, 0.0);
     if (cfg.ic_mode == 1) {
         // reference/C/LULESH-OMP.cc:3052-3053, :3112
-        m.e[0] = 3.948746e+7;
+        m.e[0] = 3.948746e+7 * cfg.efactor;
         m.deltatime0 = 1.0e-7;
         m.dtfixed = -1.0e-7;
     } else {
-        // reference/C/lulesh-2.0.3/lulesh-init.cc:140, :167-182
+        // reference/C/lulesh-2.0.3/lulesh-init.cc:140, :167-182; an ensemble
+        // member's energy factor multiplies the scaled energy last (x 1.0 is
+        // exact, so the single run is unchanged)
         const double ebase = 3.948746e+7;
         const double scale = double(nx * 1) / 45.0;
-        const double einit = ebase * scale * scale * scale;
+        const double einit = ebase * scale * scale * scale * cfg.efactor;
         m.e[0] = einit;
         m.deltatime0 = (.5 * std::cbrt(m.volo[0])) / std::sqrt(2.0 * einit);
         m.dtfixed = -1.0e-6;

@@ -20,6 +20,7 @@ struct MeshConfig {
     int num_reg = 11;
     int balance = 1;
     int cost = 1;
+    double efactor = 1.0;  // multiplies the initial origin energy
 };
 
 struct HostMesh {

@@ -42,6 +42,12 @@ class Config(ctypes.Structure):
                 ("stoptime", ctypes.c_double)]
 
 
+class Ensemble(ctypes.Structure):
+    _fields_ = [("num_members", ctypes.c_int), ("nx_min", ctypes.c_int),
+                ("nx_max", ctypes.c_int), ("efactor_min", ctypes.c_double),
+                ("efactor_max", ctypes.c_double), ("seed", ctypes.c_ulonglong)]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("time", ctypes.c_double), ("deltatime", ctypes.c_double),
                 ("dtcourant", ctypes.c_double), ("dthydro", ctypes.c_double),
@@ -62,6 +68,12 @@ EXPORTS = (
     "lulesh_kernel_count", "lulesh_kernel_name", "lulesh_set_kernel_timing",
     "lulesh_kernel_times", "lulesh_last_step_ms", "lulesh_last_error",
     "lulesh_device_math",
+    "lulesh_ensemble_default", "lulesh_ensemble_draw", "lulesh_batch_create",
+    "lulesh_batch_destroy", "lulesh_batch_step", "lulesh_batch_run",
+    "lulesh_batch_get_stats", "lulesh_batch_get_field",
+    "lulesh_batch_set_shape", "lulesh_batch_get_shape",
+    "lulesh_batch_autotune", "lulesh_batch_last_ms",
+    "lulesh_batch_last_error",
 )
 
 
@@ -124,6 +136,33 @@ def lib() -> ctypes.CDLL:
     L.lulesh_last_error.restype = ctypes.c_char_p
     L.lulesh_device_math.argtypes = [I, I, DP, DP, DP]
     L.lulesh_device_math.restype = I
+    L.lulesh_ensemble_default.argtypes = [ctypes.POINTER(Ensemble)]
+    L.lulesh_ensemble_default.restype = None
+    L.lulesh_ensemble_draw.argtypes = [ctypes.POINTER(Ensemble), IP, DP]
+    L.lulesh_ensemble_draw.restype = I
+    L.lulesh_batch_create.argtypes = [ctypes.POINTER(Config), DP, I,
+                                      ctypes.POINTER(P)]
+    L.lulesh_batch_create.restype = I
+    L.lulesh_batch_destroy.argtypes = [P]
+    L.lulesh_batch_destroy.restype = None
+    L.lulesh_batch_step.argtypes = [P, I, IP]
+    L.lulesh_batch_step.restype = I
+    L.lulesh_batch_run.argtypes = [P]
+    L.lulesh_batch_run.restype = I
+    L.lulesh_batch_get_stats.argtypes = [P, I, ctypes.POINTER(Stats)]
+    L.lulesh_batch_get_stats.restype = I
+    L.lulesh_batch_get_field.argtypes = [P, I, I, DP]
+    L.lulesh_batch_get_field.restype = I
+    L.lulesh_batch_set_shape.argtypes = [P] + [I] * 7
+    L.lulesh_batch_set_shape.restype = I
+    L.lulesh_batch_get_shape.argtypes = [P] + [IP] * 7
+    L.lulesh_batch_get_shape.restype = I
+    L.lulesh_batch_autotune.argtypes = [P, I, DP]
+    L.lulesh_batch_autotune.restype = I
+    L.lulesh_batch_last_ms.argtypes = [P]
+    L.lulesh_batch_last_ms.restype = D
+    L.lulesh_batch_last_error.argtypes = [P]
+    L.lulesh_batch_last_error.restype = ctypes.c_char_p
     _lib = L
     return L
 
@@ -300,3 +339,140 @@ def device_math(op: str, a, b=None) -> np.ndarray:
     if rc != OK:
         raise LuleshError(rc, "lulesh_device_math failed")
     return out
+
+
+def draw_ensemble(num_members: int = 64, nx_min: int = 16, nx_max: int = 64,
+                  efactor_min: float = 0.8, efactor_max: float = 1.2,
+                  seed: int = 0) -> tuple[np.ndarray, np.ndarray]:
+    """Edge sizes and origin-energy factors of a seeded ensemble (the
+    library's splitmix64 stream)."""
+    e = Ensemble()
+    lib().lulesh_ensemble_default(ctypes.byref(e))
+    e.num_members, e.nx_min, e.nx_max = num_members, nx_min, nx_max
+    e.efactor_min, e.efactor_max, e.seed = efactor_min, efactor_max, seed
+    nx = np.empty(num_members, np.int32)
+    ef = np.empty(num_members, np.float64)
+    rc = lib().lulesh_ensemble_draw(
+        ctypes.byref(e), nx.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(ef))
+    if rc != OK:
+        raise LuleshError(rc, "lulesh_ensemble_draw failed")
+    return nx, ef
+
+
+class Batch:
+    """An ensemble of independent Sedov domains advanced together.
+
+    Member i is the domain Domain(nx[i], ...) would build, with its initial
+    origin energy multiplied by efactor[i]; every member keeps its own time,
+    dt and stop test.
+    """
+
+    def __init__(self, nx, efactor=None, ic_mode: int = 2, num_reg: int = 11,
+                 balance: int = 1, cost: int = 1, max_cycles: int = 9999999,
+                 stoptime: float = 1.0e-2):
+        L = lib()
+        nx = [int(v) for v in nx]
+        n = len(nx)
+        cfgs = (Config * n)()
+        for i, v in enumerate(nx):
+            L.lulesh_config_default(ctypes.byref(cfgs[i]))
+            c = cfgs[i]
+            c.nx, c.ic_mode, c.num_reg, c.balance, c.cost = v, ic_mode, num_reg, balance, cost
+            c.max_cycles, c.stoptime = max_cycles, stoptime
+        ef = None
+        if efactor is not None:
+            self._ef = np.ascontiguousarray(efactor, np.float64)
+            if self._ef.size != n:
+                raise ValueError("efactor needs one value per member")
+            ef = _dp(self._ef)
+        h = ctypes.c_void_p()
+        rc = L.lulesh_batch_create(cfgs, ef, n, ctypes.byref(h))
+        if rc != OK:
+            raise LuleshError(rc, "lulesh_batch_create failed")
+        self.h = h
+        self.nx = np.array(nx)
+        self.n = n
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().lulesh_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int) -> int:
+        if rc != OK:
+            raise LuleshError(rc, lib().lulesh_batch_last_error(self.h).decode())
+        return rc
+
+    def step(self, ncycles: int) -> int:
+        """Advances every unfinished member by up to ncycles cycles; returns
+        the number of members with cycles left."""
+        p = ctypes.c_int(0)
+        self._check(lib().lulesh_batch_step(self.h, int(ncycles), ctypes.byref(p)))
+        return p.value
+
+    def run(self, raise_on_error: bool = True) -> int:
+        rc = lib().lulesh_batch_run(self.h)
+        if raise_on_error:
+            self._check(rc)
+        return rc
+
+    def stats(self, member: int) -> tuple[int, Stats]:
+        """(error code of the member, its scalars)."""
+        s = Stats()
+        rc = lib().lulesh_batch_get_stats(self.h, int(member), ctypes.byref(s))
+        if rc == BAD_ARG:
+            self._check(rc)
+        return rc, s
+
+    def get(self, member: int, name: str) -> np.ndarray:
+        _, st = self.stats(member)
+        if name in NODE_FIELDS:
+            fid, n = NODE_FIELDS[name], st.num_node
+        else:
+            fid, n = ELEM_FIELDS[name], st.num_elem
+        out = np.empty(n, np.float64)
+        self._check(lib().lulesh_batch_get_field(self.h, int(member), fid, _dp(out)))
+        return out
+
+    def set_shape(self, tx: int, ty: int, nz: int, waves: int, sweep_blocks: int = 0,
+                  elem_blocks: int = 0, cycles_per_graph: int = 16) -> None:
+        self._check(lib().lulesh_batch_set_shape(self.h, tx, ty, nz, waves,
+                                                 sweep_blocks, elem_blocks,
+                                                 cycles_per_graph))
+
+    def shape(self) -> dict:
+        v = [ctypes.c_int() for _ in range(7)]
+        self._check(lib().lulesh_batch_get_shape(self.h, *[ctypes.byref(a) for a in v]))
+        keys = ("tx", "ty", "nz", "waves", "sweep_blocks", "elem_blocks",
+                "cycles_per_graph")
+        return dict(zip(keys, (a.value for a in v)))
+
+    def autotune(self, trial_cycles: int = 16) -> float:
+        """Picks the fastest launch shape; returns its ms per cycle."""
+        ms = ctypes.c_double(0.0)
+        self._check(lib().lulesh_batch_autotune(self.h, int(trial_cycles),
+                                                ctypes.byref(ms)))
+        return ms.value
+
+    def last_ms(self) -> float:
+        return lib().lulesh_batch_last_ms(self.h)
+
+    def zone_cycles(self) -> int:
+        """Sum over members of elements x cycles done."""
+        tot = 0
+        for i in range(self.n):
+            _, st = self.stats(i)
+            tot += int(st.num_elem) * int(st.cycle)
+        return tot

@@ -44,6 +44,7 @@ enum { LO_VOLUME_ERROR = -1, LO_QSTOP_ERROR = -2 };
 struct lo_domain {
     int nx, mode, num_elem, num_node, edge_nodes;
     int num_reg, balance, cost, cbrt_mode;
+    double efactor; /* multiplies the initial origin energy (ensembles) */
 
     /* node-centred */
     double *x, *y, *z, *xd, *yd, *zd, *xdd, *ydd, *zdd, *fx, *fy, *fz;
@@ -1222,9 +1223,16 @@ static void create_regions(lo_domain *d)
 
 lo_domain *lo_create(int nx, int mode, int num_reg, int balance, int cost)
 {
-    if (nx < 1 || (mode != 1 && mode != 2)) return NULL;
+    return lo_create_scaled(nx, mode, num_reg, balance, cost, 1.0);
+}
+
+lo_domain *lo_create_scaled(int nx, int mode, int num_reg, int balance, int cost,
+                            double efactor)
+{
+    if (nx < 1 || (mode != 1 && mode != 2) || !(efactor > 0.0)) return NULL;
     lo_domain *d = (lo_domain *)calloc(1, sizeof(lo_domain));
     const int en = nx + 1;
+    d->efactor = efactor;
     d->nx = nx;
     d->mode = mode;
     d->edge_nodes = en;
@@ -1373,12 +1381,12 @@ lo_domain *lo_create(int nx, int mode, int num_reg, int balance, int cost)
     /* initial energy and time step: 1.0 (:3053, :3112); 2.0 scales the
      * energy by (nx/45)^3 and derives dt from it (lulesh-init.cc:167-182) */
     if (mode == 1) {
-        d->e[0] = 3.948746e+7;
+        d->e[0] = 3.948746e+7 * d->efactor;
         d->deltatime = 1.0e-7;
     } else {
         const double ebase = 3.948746e+7;
         const double scale = (double)(nx * 1) / 45.0;
-        const double einit = ebase * scale * scale * scale;
+        const double einit = ebase * scale * scale * scale * d->efactor;
         d->e[0] = einit;
         d->deltatime = (.5 * cbrt(d->volo[0])) / sqrt(2.0 * einit);
     }

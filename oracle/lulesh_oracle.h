@@ -28,6 +28,11 @@ typedef struct lo_domain lo_domain;
 /* mode: 1 = LULESH 1.0 (LULESH-OMP.cc), 2 = LULESH 2.0.3.
  * num_reg/balance/cost are the 2.0 -r/-b/-c options (ignored in mode 1). */
 lo_domain *lo_create(int nx, int mode, int num_reg, int balance, int cost);
+/* An ensemble member: the initial origin energy (and, in mode 2, the
+ * initial dt derived from it) is that of lo_create times efactor, applied
+ * as the last factor of the energy product. */
+lo_domain *lo_create_scaled(int nx, int mode, int num_reg, int balance, int cost,
+                            double efactor);
 void lo_destroy(lo_domain *d);
 
 /* cube root used by the hourglass coefficient: 0 = C library cbrt (the

@@ -48,6 +48,8 @@ def lib() -> ctypes.CDLL:
         I = ctypes.POINTER(ctypes.c_int)
         L.lo_create.restype = P
         L.lo_create.argtypes = [ctypes.c_int] * 5
+        L.lo_create_scaled.restype = P
+        L.lo_create_scaled.argtypes = [ctypes.c_int] * 5 + [ctypes.c_double]
         L.lo_destroy.argtypes = [P]
         L.lo_set_cbrt_mode.argtypes = [P, ctypes.c_int]
         L.lo_set_stoptime.argtypes = [P, ctypes.c_double]
@@ -100,10 +102,11 @@ class Oracle:
 
     def __init__(self, nx: int, mode: int = 2, num_reg: int = 11,
                  balance: int = 1, cost: int = 1, cbrt_mode: int = 0,
-                 stoptime: float | None = None, threads: int = 0):
+                 stoptime: float | None = None, threads: int = 0,
+                 efactor: float = 1.0):
         L = lib()
         L.lo_set_threads(threads)
-        self.h = L.lo_create(nx, mode, num_reg, balance, cost)
+        self.h = L.lo_create_scaled(nx, mode, num_reg, balance, cost, efactor)
         if not self.h:
             raise ValueError("lo_create failed")
         self.nx, self.mode = nx, mode
