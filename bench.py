@@ -13,7 +13,13 @@ from HIP events on the domain's stream) and the reference LULESH 2.0.3
 (oracle/_ref/lulesh2, built from the reference sources) timed on this
 host's cores on a 20-cycle sample of the same workload.
 
+The "forms" object holds the other forms of the step on their own
+workloads: the ensemble (64 Sedov members of uneven size, each to t=1e-2)
+and the mixed-precision step (120^3 to t=1e-2, with its errors against the
+float64 run).
+
     python bench.py [--steps K] [--warmup W] [--nx N] [--no-cpu-baseline]
+                    [--no-forms]
 """
 from __future__ import annotations
 
@@ -107,6 +113,97 @@ def load_traffic(kernel: str, nx: int) -> float | None:
         return None
 
 
+def bench_batched(lulesh_amd, plain_value: float) -> dict:
+    """The ensemble form on its workload: 64 Sedov members, nx uniform in
+    [16, 64] and origin-energy factors uniform in [0.8, 1.2] (splitmix64,
+    seed 0), LULESH 2.0 ICs with 11 regions, every member run to t = 1e-2
+    with its own dt.  zones*cycles counts each member's own cycles only.
+    Beside it: the same members' sizes run one after another through the
+    single-domain path (energy factor 1), the way N separate runs would."""
+    nx, ef = lulesh_amd.draw_ensemble()
+    t0 = time.perf_counter()
+    b = lulesh_amd.Batch(nx, ef)
+    t_build = time.perf_counter() - t0
+    print(f"bench: ensemble built in {t_build:.1f} s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    tuned_ms = b.autotune(16)
+    t_tune = time.perf_counter() - t0
+    shape = b.shape()
+    print(f"bench: autotuned in {t_tune:.1f} s: {shape}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    rc = b.run()
+    wall = time.perf_counter() - t0
+    zc = b.zone_cycles()
+    cycles = [b.stats(i)[1].cycle for i in range(b.n)]
+    b.close()
+    print(f"bench: ensemble ran in {wall:.2f} s", file=sys.stderr, flush=True)
+    zc_plain, wall_plain = 0, 0.0
+    for n in nx:
+        d = lulesh_amd.Domain(int(n))
+        t0 = time.perf_counter()
+        d.run()
+        wall_plain += time.perf_counter() - t0
+        zc_plain += int(n) ** 3 * d.stats().cycle
+        d.close()
+    print(f"bench: single-domain runs took {wall_plain:.2f} s", file=sys.stderr,
+          flush=True)
+    value = zc / wall
+    return {
+        "value": value, "unit": "zones·cycles/s", "rc": rc,
+        "workload": "ensemble of 64 Sedov members, nx~U{16..64}, "
+                    "efactor~U[0.8,1.2], splitmix64 seed 0, LULESH 2.0 ICs, "
+                    "11 regions, each to t=1e-2",
+        "members": int(len(nx)), "zones_total": int(sum(int(n) ** 3 for n in nx)),
+        "zone_cycles": zc, "member_cycles": [int(min(cycles)), int(max(cycles))],
+        "wall_s": wall, "build_s": t_build, "autotune_s": t_tune,
+        "autotune_ms_per_cycle": tuned_ms, "shape": shape,
+        "same_sizes_single_domain_path": {"value": zc_plain / wall_plain,
+                                          "wall_s": wall_plain},
+        "vs_single_domain_path": value / (zc_plain / wall_plain),
+        "vs_plain_150": value / plain_value,
+    }
+
+
+def bench_mixed(lulesh_amd, nx: int = 120) -> dict:
+    """The mixed-precision form on its workload: Sedov at nx^3 (LULESH 2.0
+    ICs, 11 regions), run to t = 1e-2 in float64 and in mixed precision;
+    throughput of both over the whole run and the mixed run's errors
+    against the float64 one (origin energy, total energy, shock radius)."""
+    out = {}
+    doms = {}
+    for name, mixed in (("fp64", False), ("mixed", True)):
+        d = lulesh_amd.Domain(nx, mixed=mixed)
+        d.step(1)  # first launch of each kernel outside the timed run
+        t0 = time.perf_counter()
+        rc, n = d.run()
+        wall = time.perf_counter() - t0
+        st = d.stats()
+        out[name] = {"value": nx ** 3 * n / wall, "cycles": st.cycle, "wall_s": wall}
+        doms[name] = d
+        print(f"bench: {name} {nx}^3 to t=1e-2: {st.cycle} cycles in {wall:.2f} s",
+              file=sys.stderr, flush=True)
+    r, m = doms["fp64"], doms["mixed"]
+    e_r, e_m = r.stats().e_origin, m.stats().e_origin
+    t_r, t_m = r.total_energy(), m.total_energy()
+    s_r, s_m = r.shock_radius(), m.shock_radius()
+    for d in doms.values():
+        d.close()
+    return {
+        "value": out["mixed"]["value"], "unit": "zones·cycles/s",
+        "workload": f"sedov_{nx}^3_lulesh2.0_regions11 to t=1e-2",
+        "fp64": out["fp64"], "mixed": out["mixed"],
+        "vs_fp64_same_workload": out["mixed"]["value"] / out["fp64"]["value"],
+        "errors_vs_fp64": {
+            "origin_energy_rel": abs(e_m - e_r) / abs(e_r),
+            "total_energy_rel": abs(t_m - t_r) / abs(t_r),
+            "shock_radius_abs": abs(s_m["radius"] - s_r["radius"]),
+            "shock_radius_x_axis_abs": abs(s_m["radius_x_axis"] - s_r["radius_x_axis"]),
+            "element_size": 1.125 / nx,
+            "cycles": [out["fp64"]["cycles"], out["mixed"]["cycles"]],
+        },
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +212,8 @@ def main() -> int:
     ap.add_argument("--nx", type=int, default=150)
     ap.add_argument("--cpu-cycles", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the ensemble / mixed / learned-Q forms")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not "
@@ -182,6 +281,11 @@ def main() -> int:
         "kernels_mean_ms": {k: round(v, 6) for k, v in per_kernel.items()},
         "gpu_event_ms_per_step": sum(ms for ms, _ in times.values()) / K,
     }
+    if not args.no_forms:
+        result["forms"] = {"batched": bench_batched(lulesh_amd, value),
+                           "mixed": bench_mixed(lulesh_amd)}
+        result["forms"]["mixed"]["vs_plain_150"] = (
+            result["forms"]["mixed"]["value"] / value)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(nx, args.cpu_cycles)
     print(json.dumps(result))

@@ -90,6 +90,19 @@ void lulesh_config_default(lulesh_config *cfg);
 int lulesh_create(const lulesh_config *cfg, lulesh_domain **out);
 void lulesh_destroy(lulesh_domain *d);
 
+/* The mixed-precision form of the same step (no counterpart in the
+ * reference, which is float64 throughout): the element-local arithmetic of
+ * the stress integration, hourglass control, monotonic Q and EOS runs in
+ * float32, while node coordinates and velocities, element volumes (the
+ * Jacobian determinants), the energy update and the dt reductions stay
+ * float64.  Every other entry point works on such a domain unchanged; the
+ * float32 fields are widened to double on the way out and rounded on the
+ * way in.  Its results differ from lulesh_create's by rounding: the origin
+ * energy at t = 1e-2 within 1e-5 relative at 120^3. */
+int lulesh_create_mixed(const lulesh_config *cfg, lulesh_domain **out);
+/* 1 for a domain made by lulesh_create_mixed, else 0. */
+int lulesh_is_mixed(const lulesh_domain *d);
+
 /* Advances up to ncycles cycles, stopping early at stoptime/max_cycles or on
  * an error; *cycles_done (may be NULL) receives the cycles completed.
  * Replaces one or more turns of the reference loop
@@ -234,6 +247,38 @@ int lulesh_batch_autotune(lulesh_batch *b, int trial_cycles, double *best_ms);
 /* Milliseconds between the HIP events around the last step/run call. */
 double lulesh_batch_last_ms(lulesh_batch *b);
 const char *lulesh_batch_last_error(const lulesh_batch *b);
+
+/* ------------------------------------------------------------------------
+ * The learned-Q network (no counterpart in the reference): a 12-64-64-2
+ * multilayer perceptron with ReLU hidden layers, bf16 weights and
+ * activations with fp32 accumulation on the matrix cores, fp32 master
+ * weights and Adam for training.  Parameters, fp32, in the order W1[12][64]
+ * (input-major), b1[64], W2[64][64], b2[64], W3[64][2], b3[2]
+ * (lulesh_mlp_param_count() values).  Inputs are standardised on the device
+ * with the given per-input mean and standard deviation.
+ * ------------------------------------------------------------------------ */
+#define LULESH_MLP_INPUTS 12
+#define LULESH_MLP_OUTPUTS 2
+typedef struct lulesh_mlp lulesh_mlp;
+
+int lulesh_mlp_param_count(void);
+int lulesh_mlp_create(const float *params, const float *in_mean, const float *in_std,
+                      lulesh_mlp **out);
+void lulesh_mlp_destroy(lulesh_mlp *n);
+int lulesh_mlp_get_params(lulesh_mlp *n, float *params);
+/* Outputs y[count][2] of the raw inputs x[count][12]. */
+int lulesh_mlp_eval(lulesh_mlp *n, const float *x, int count, float *y);
+/* Gradient of the loss 0.5/count * sum_s sum_o w_o (y_so - t_so)^2 with
+ * respect to the parameters (w = out_weight[2], NULL: 1), and the loss. */
+int lulesh_mlp_gradient(lulesh_mlp *n, const float *x, const float *t, int count,
+                        const float *out_weight, float *grad, double *loss);
+/* Adam (beta 0.9 / 0.999, eps 1e-8) on minibatches of `batch` samples drawn
+ * without replacement (splitmix64 shuffle from `seed`), `epochs` passes;
+ * *final_loss receives the mean minibatch loss of the last pass. */
+int lulesh_mlp_train(lulesh_mlp *n, const float *x, const float *t, int count,
+                     const float *out_weight, int epochs, int batch, float lr,
+                     unsigned long long seed, double *final_loss);
+const char *lulesh_mlp_last_error(const lulesh_mlp *n);
 
 #ifdef __cplusplus
 }

@@ -21,10 +21,12 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "liblulesh_mi355x.so")
 HARNESS = os.path.join(PKG, "lulesh_mi355x")
 
-HIP_SOURCES = ["lulesh_kernels.hip", "lulesh_force.hip", "lulesh_abi.hip",
-               "lulesh_batch.hip", "lulesh_setup.cpp", "lulesh_mesh.cpp"]
+HIP_SOURCES = ["lulesh_kernels.hip", "lulesh_force.hip", "lulesh_mixed.hip",
+               "lulesh_abi.hip", "lulesh_batch.hip", "lulesh_mlp.hip",
+               "lulesh_setup.cpp",
+               "lulesh_mesh.cpp"]
 HEADERS = ["lulesh_kernels.hpp", "lulesh_elem.hpp", "lulesh_cbrt.h",
-           "lulesh_mesh.hpp", "lulesh_setup.hpp"]
+           "lulesh_mesh.hpp", "lulesh_setup.hpp", "lulesh_mlp.hpp"]
 OBJ_DIR = os.path.join(PKG, "_build")
 
 COMMON_FLAGS = [

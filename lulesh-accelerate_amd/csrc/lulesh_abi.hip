@@ -40,6 +40,9 @@ const char *const kKernelNames[KID_COUNT] = {
 // planes per block, register budget for 2 waves per SIMD);
 // lulesh_set_sweep_shape overrides it
 constexpr SweepShape kDefaultSweep = {32, 8, 16, 2};
+// the mixed-precision sweep holds float32 corner forces: three blocks fit
+// on a CU
+constexpr SweepShape kDefaultSweepMixed = {32, 8, 12, 3};
 
 constexpr int kBlock = 256;
 constexpr int kBatch = 32;  // cycles queued per host round trip in run()
@@ -58,7 +61,9 @@ struct lulesh_domain {
     int num_reg = 1;
     std::vector<int32_t> reg_num_list;
 
+    bool mixed = false;   // float32 element-local arithmetic (fm), else f
     DevFields f{};
+    DevFieldsMixed fm{};
     DevScalars *s = nullptr;
     PhysParams prm{};
     SweepShape sweep = kDefaultSweep;
@@ -136,22 +141,38 @@ int queue_cycle(lulesh_domain *d)
     hipStream_t st = d->stream;
     DevScalars *s = d->s;
     const DevFields &f = d->f;
+    const DevFieldsMixed &fm = d->fm;
     const PhysParams &p = d->prm;
+    const bool mixed = d->mixed;
     int rc;
     if ((rc = launch(d, KID_TIME_INCREMENT, [&] {
              hipLaunchKernelGGL(k_time_increment, dim3(1), dim3(64), 0, st, s, p.ic_mode);
          })))
         return rc;
     if ((rc = launch(d, KID_FORCE_SWEEP, [&] {
-             (void)launch_force_sweep(d->sweep, f, s, p, st);
+             if (mixed) {
+                 (void)launch_force_sweep_mixed(d->sweep, fm, s, p, st);
+             } else {
+                 (void)launch_force_sweep(d->sweep, f, s, p, st);
+             }
          })))
         return rc;
     if ((rc = launch(d, KID_KINEMATICS_Q, [&] {
-             hipLaunchKernelGGL(k_kinematics_q, dim3(ge), dim3(kBlock), 0, st, f, s, p);
+             if (mixed) {
+                 hipLaunchKernelGGL(k_kinematics_q_mixed, dim3(ge), dim3(kBlock), 0, st, fm,
+                                    s, p);
+             } else {
+                 hipLaunchKernelGGL(k_kinematics_q, dim3(ge), dim3(kBlock), 0, st, f, s, p);
+             }
          })))
         return rc;
     if ((rc = launch(d, KID_Q_EOS_DT, [&] {
-             hipLaunchKernelGGL(k_q_eos_dt, dim3(ge), dim3(kBlock), 0, st, f, s, p);
+             if (mixed) {
+                 hipLaunchKernelGGL(k_q_eos_dt_mixed, dim3(ge), dim3(kBlock), 0, st, fm, s,
+                                    p);
+             } else {
+                 hipLaunchKernelGGL(k_q_eos_dt, dim3(ge), dim3(kBlock), 0, st, f, s, p);
+             }
          })))
         return rc;
     return LULESH_OK;
@@ -188,39 +209,80 @@ int collect_timings(lulesh_domain *d)
 }
 
 struct FieldRef {
-    double *ptr;
+    void *ptr;
     int64_t n;
+    bool f32;  // stored as float32 (mixed precision)
 };
 
 // par: the copy of the node arrays that holds the current state
 FieldRef field_ref(lulesh_domain *d, int id, int par)
 {
-    int64_t n = 0;
-    double *p = field_pointer(d->f, id, par, &n);
-    return {p, n};
+    FieldRef r{nullptr, 0, false};
+    if (d->mixed) {
+        r.ptr = field_pointer_mixed(d->fm, id, par, &r.n, &r.f32);
+    } else {
+        r.ptr = field_pointer(d->f, id, par, &r.n);
+    }
+    return r;
 }
 
-int copy_in(lulesh_domain *d, double *dst, const double *src, int64_t n)
+// host float64 <-> device field (converted when the field is float32);
+// synchronous
+int copy_in(lulesh_domain *d, const FieldRef &r, const double *src)
 {
-    HIP_TRY(d, hipMemcpyAsync(dst, src, size_t(n) * sizeof(double),
+    if (!r.f32) {
+        HIP_TRY(d, hipMemcpyAsync(r.ptr, src, size_t(r.n) * sizeof(double),
+                                  hipMemcpyHostToDevice, d->stream));
+        HIP_TRY(d, hipStreamSynchronize(d->stream));
+        return LULESH_OK;
+    }
+    std::vector<float> tmp(static_cast<size_t>(r.n));
+    for (int64_t i = 0; i < r.n; ++i) {
+        tmp[size_t(i)] = static_cast<float>(src[i]);
+    }
+    HIP_TRY(d, hipMemcpyAsync(r.ptr, tmp.data(), tmp.size() * sizeof(float),
                               hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(d, hipStreamSynchronize(d->stream));
     return LULESH_OK;
 }
 
-int copy_out(lulesh_domain *d, double *dst, const double *src, int64_t n)
+int copy_out(lulesh_domain *d, double *dst, const FieldRef &r)
 {
-    HIP_TRY(d, hipMemcpyAsync(dst, src, size_t(n) * sizeof(double),
+    if (!r.f32) {
+        HIP_TRY(d, hipMemcpyAsync(dst, r.ptr, size_t(r.n) * sizeof(double),
+                                  hipMemcpyDeviceToHost, d->stream));
+        HIP_TRY(d, hipStreamSynchronize(d->stream));
+        return LULESH_OK;
+    }
+    std::vector<float> tmp(static_cast<size_t>(r.n));
+    HIP_TRY(d, hipMemcpyAsync(tmp.data(), r.ptr, tmp.size() * sizeof(float),
                               hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(d, hipStreamSynchronize(d->stream));
+    for (int64_t i = 0; i < r.n; ++i) {
+        dst[i] = static_cast<double>(tmp[size_t(i)]);
+    }
     return LULESH_OK;
 }
+
+// the domain tuple's fields in the order of upload/download_state
+constexpr int kStateFields[11] = {LULESH_F_X, LULESH_F_Y, LULESH_F_Z, LULESH_F_XD,
+                                  LULESH_F_YD, LULESH_F_ZD, LULESH_F_E, LULESH_F_P,
+                                  LULESH_F_Q, LULESH_F_V, LULESH_F_SS};
+
+double *energy_ptr(lulesh_domain *d) { return d->mixed ? d->fm.e : d->f.e; }
 
 int build_device_domain(lulesh_domain *d, const HostMesh &m)
 {
-    const size_t bytes = domain_arena_bytes(m);
+    const size_t bytes = d->mixed ? domain_arena_bytes_mixed(m) : domain_arena_bytes(m);
     HIP_TRY(d, hipMalloc(&d->arena, bytes));
     HIP_TRY(d, hipMemsetAsync(d->arena, 0, bytes, d->stream));
-    carve_domain(static_cast<char *>(d->arena), m, d->f, &d->s);
-    HIP_TRY(d, upload_mesh(m, d->f, d->stream));
+    if (d->mixed) {
+        carve_domain_mixed(static_cast<char *>(d->arena), m, d->fm, &d->s);
+        HIP_TRY(d, upload_mesh_mixed(m, d->fm, d->stream));
+    } else {
+        carve_domain(static_cast<char *>(d->arena), m, d->f, &d->s);
+        HIP_TRY(d, upload_mesh(m, d->f, d->stream));
+    }
     return write_scalars(d, initial_scalars(m, d->cfg.stoptime, d->cfg.max_cycles));
 }
 
@@ -242,7 +304,11 @@ void lulesh_config_default(lulesh_config *cfg)
     cfg->stoptime = 1.0e-2;
 }
 
-int lulesh_create(const lulesh_config *cfg, lulesh_domain **out)
+}  // extern "C"
+
+namespace {
+
+int create_domain(const lulesh_config *cfg, bool mixed, lulesh_domain **out)
 {
     if (!cfg || !out) {
         return LULESH_BAD_ARG;
@@ -256,6 +322,8 @@ int lulesh_create(const lulesh_config *cfg, lulesh_domain **out)
         return LULESH_BAD_ARG;
     }
     d->cfg = *cfg;
+    d->mixed = mixed;
+    d->sweep = mixed ? kDefaultSweepMixed : kDefaultSweep;
     HostMesh m;
     try {
         MeshConfig mc;
@@ -294,6 +362,22 @@ int lulesh_create(const lulesh_config *cfg, lulesh_domain **out)
     *out = d;
     return LULESH_OK;
 }
+
+}  // namespace
+
+extern "C" {
+
+int lulesh_create(const lulesh_config *cfg, lulesh_domain **out)
+{
+    return create_domain(cfg, false, out);
+}
+
+int lulesh_create_mixed(const lulesh_config *cfg, lulesh_domain **out)
+{
+    return create_domain(cfg, true, out);
+}
+
+int lulesh_is_mixed(const lulesh_domain *d) { return (d && d->mixed) ? 1 : 0; }
 
 void lulesh_destroy(lulesh_domain *d)
 {
@@ -389,19 +473,13 @@ int lulesh_upload_state(lulesh_domain *d, const double *x, const double *y,
     if (!d || !x || !y || !z || !xd || !yd || !zd || !e || !p || !q || !v || !ss) {
         return LULESH_BAD_ARG;
     }
-    const DevFields &f = d->f;
-    const int64_t nn = d->num_node, ne = d->num_elem;
     DevScalars s{};
     int rc = read_scalars(d, &s);
     if (rc) return rc;
-    const int c = s.cur;
-    if ((rc = copy_in(d, f.x[c], x, nn)) || (rc = copy_in(d, f.y[c], y, nn)) ||
-        (rc = copy_in(d, f.z[c], z, nn)) || (rc = copy_in(d, f.xd[c], xd, nn)) ||
-        (rc = copy_in(d, f.yd[c], yd, nn)) || (rc = copy_in(d, f.zd[c], zd, nn)) ||
-        (rc = copy_in(d, f.e, e, ne)) || (rc = copy_in(d, f.p, p, ne)) ||
-        (rc = copy_in(d, f.q, q, ne)) || (rc = copy_in(d, f.v, v, ne)) ||
-        (rc = copy_in(d, f.ss, ss, ne)))
-        return rc;
+    const double *src[11] = {x, y, z, xd, yd, zd, e, p, q, v, ss};
+    for (int i = 0; i < 11; ++i) {
+        if ((rc = copy_in(d, field_ref(d, kStateFields[i], s.cur), src[i]))) return rc;
+    }
     return lulesh_set_scalars(d, &time, &deltatime, nullptr, nullptr, nullptr);
 }
 
@@ -413,20 +491,13 @@ int lulesh_download_state(lulesh_domain *d, double *x, double *y, double *z,
     if (!d || !x || !y || !z || !xd || !yd || !zd || !e || !p || !q || !v || !ss) {
         return LULESH_BAD_ARG;
     }
-    const DevFields &f = d->f;
-    const int64_t nn = d->num_node, ne = d->num_elem;
     DevScalars s{};
     int rc = read_scalars(d, &s);
     if (rc) return rc;
-    const int c = s.cur;
-    if ((rc = copy_out(d, x, f.x[c], nn)) || (rc = copy_out(d, y, f.y[c], nn)) ||
-        (rc = copy_out(d, z, f.z[c], nn)) || (rc = copy_out(d, xd, f.xd[c], nn)) ||
-        (rc = copy_out(d, yd, f.yd[c], nn)) || (rc = copy_out(d, zd, f.zd[c], nn)) ||
-        (rc = copy_out(d, e, f.e, ne)) || (rc = copy_out(d, p, f.p, ne)) ||
-        (rc = copy_out(d, q, f.q, ne)) || (rc = copy_out(d, v, f.v, ne)) ||
-        (rc = copy_out(d, ss, f.ss, ne)))
-        return rc;
-    HIP_TRY(d, hipStreamSynchronize(d->stream));
+    double *dst[11] = {x, y, z, xd, yd, zd, e, p, q, v, ss};
+    for (int i = 0; i < 11; ++i) {
+        if ((rc = copy_out(d, dst[i], field_ref(d, kStateFields[i], s.cur)))) return rc;
+    }
     if (time) *time = s.time;
     if (deltatime) *deltatime = s.deltatime;
     return LULESH_OK;
@@ -444,10 +515,7 @@ int lulesh_get_field(lulesh_domain *d, int field, double *host_out)
     if (!r.ptr) {
         return fail(d, LULESH_BAD_ARG, "unknown field id");
     }
-    rc = copy_out(d, host_out, r.ptr, r.n);
-    if (rc) return rc;
-    HIP_TRY(d, hipStreamSynchronize(d->stream));
-    return LULESH_OK;
+    return copy_out(d, host_out, r);
 }
 
 int lulesh_set_field(lulesh_domain *d, int field, const double *host_in)
@@ -462,10 +530,7 @@ int lulesh_set_field(lulesh_domain *d, int field, const double *host_in)
     if (!r.ptr) {
         return fail(d, LULESH_BAD_ARG, "unknown field id");
     }
-    rc = copy_in(d, r.ptr, host_in, r.n);
-    if (rc) return rc;
-    HIP_TRY(d, hipStreamSynchronize(d->stream));
-    return LULESH_OK;
+    return copy_in(d, r, host_in);
 }
 
 int lulesh_get_stats(lulesh_domain *d, lulesh_stats *out)
@@ -477,7 +542,7 @@ int lulesh_get_stats(lulesh_domain *d, lulesh_stats *out)
     int rc = read_scalars(d, &s);
     if (rc) return rc;
     double e0 = 0.0;
-    HIP_TRY(d, hipMemcpy(&e0, d->f.e, sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(d, hipMemcpy(&e0, energy_ptr(d), sizeof(double), hipMemcpyDeviceToHost));
     out->time = s.time;
     out->deltatime = s.deltatime;
     out->dtcourant = s.dtcourant;
@@ -555,7 +620,8 @@ int lulesh_set_sweep_shape(lulesh_domain *d, int tx, int ty, int nz, int waves)
         return LULESH_BAD_ARG;
     }
     const SweepShape sh = {tx, ty, nz, waves};
-    if (!force_sweep_shape_supported(sh)) {
+    if (!(d->mixed ? force_sweep_mixed_shape_supported(sh)
+                   : force_sweep_shape_supported(sh))) {
         return fail(d, LULESH_BAD_ARG, "sweep shape not built into the library");
     }
     d->sweep = sh;
@@ -592,7 +658,7 @@ int lulesh_symmetry(lulesh_domain *d, double *max_abs_diff,
     }
     const int nx = d->nx;
     std::vector<double> e(size_t(nx) * nx);
-    HIP_TRY(d, hipMemcpy(e.data(), d->f.e, e.size() * sizeof(double),
+    HIP_TRY(d, hipMemcpy(e.data(), energy_ptr(d), e.size() * sizeof(double),
                          hipMemcpyDeviceToHost));
     double mabs = 0.0, tabs = 0.0, mrel = 0.0;
     for (int j = 0; j < nx; ++j) {

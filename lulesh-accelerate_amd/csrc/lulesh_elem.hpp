@@ -11,7 +11,14 @@
 
 #include <hip/hip_runtime.h>
 
+// The mixed-precision translation unit defines LULESH_MIXED_TU: its results
+// are not bit-matched, so there the element templates may fuse
+// multiply-adds.  Everywhere else contraction stays off.
+#ifdef LULESH_MIXED_TU
+#pragma clang fp contract(fast)
+#else
 #pragma clang fp contract(off)
+#endif
 
 #define LULESH_FN static __device__ __forceinline__
 #include "lulesh_cbrt.h"
@@ -22,54 +29,64 @@ namespace lulesh {
 // On the structured cube they follow from the element's (i, j, k): the
 // x = 0, y = 0, z = 0 faces are symmetry planes, the far faces free surfaces.
 
+// The geometric helpers are templates on the arithmetic type T: the FP64
+// step instantiates them with double (every constant T(c) is then the
+// reference's double constant, so the rounding is unchanged), the
+// mixed-precision step with float on element-local coordinates.
+
+// fast square root of the arithmetic type: IEEE sqrt in double, the
+// hardware v_sqrt_f32 (1 ulp) in float
+__device__ __forceinline__ double elem_sqrt(double v) { return sqrt(v); }
+__device__ __forceinline__ float elem_sqrt(float v) { return __builtin_amdgcn_sqrtf(v); }
+
 // reference/C/LULESH-OMP.cc:620-706, the Jacobian determinant only (the
 // shape-function derivatives of the stress integration are overwritten by
 // the node normals before use, :884-888)
-__device__ __forceinline__ double jacobian_volume(const double *x,
-                                                  const double *y,
-                                                  const double *z)
+template <typename T>
+__device__ __forceinline__ T jacobian_volume(const T *x, const T *y, const T *z)
 {
-    const double fjxxi = .125 * ((x[6] - x[0]) + (x[5] - x[3]) - (x[7] - x[1]) - (x[4] - x[2]));
-    const double fjxet = .125 * ((x[6] - x[0]) - (x[5] - x[3]) + (x[7] - x[1]) - (x[4] - x[2]));
-    const double fjxze = .125 * ((x[6] - x[0]) + (x[5] - x[3]) + (x[7] - x[1]) + (x[4] - x[2]));
-    const double fjyxi = .125 * ((y[6] - y[0]) + (y[5] - y[3]) - (y[7] - y[1]) - (y[4] - y[2]));
-    const double fjyet = .125 * ((y[6] - y[0]) - (y[5] - y[3]) + (y[7] - y[1]) - (y[4] - y[2]));
-    const double fjyze = .125 * ((y[6] - y[0]) + (y[5] - y[3]) + (y[7] - y[1]) + (y[4] - y[2]));
-    const double fjzxi = .125 * ((z[6] - z[0]) + (z[5] - z[3]) - (z[7] - z[1]) - (z[4] - z[2]));
-    const double fjzet = .125 * ((z[6] - z[0]) - (z[5] - z[3]) + (z[7] - z[1]) - (z[4] - z[2]));
-    const double fjzze = .125 * ((z[6] - z[0]) + (z[5] - z[3]) + (z[7] - z[1]) + (z[4] - z[2]));
-    const double cjxet = -(fjyxi * fjzze) + (fjzxi * fjyze);
-    const double cjyet = (fjxxi * fjzze) - (fjzxi * fjxze);
-    const double cjzet = -(fjxxi * fjyze) + (fjyxi * fjxze);
-    return 8. * (fjxet * cjxet + fjyet * cjyet + fjzet * cjzet);
+    const T e = T(.125);
+    const T fjxxi = e * ((x[6] - x[0]) + (x[5] - x[3]) - (x[7] - x[1]) - (x[4] - x[2]));
+    const T fjxet = e * ((x[6] - x[0]) - (x[5] - x[3]) + (x[7] - x[1]) - (x[4] - x[2]));
+    const T fjxze = e * ((x[6] - x[0]) + (x[5] - x[3]) + (x[7] - x[1]) + (x[4] - x[2]));
+    const T fjyxi = e * ((y[6] - y[0]) + (y[5] - y[3]) - (y[7] - y[1]) - (y[4] - y[2]));
+    const T fjyet = e * ((y[6] - y[0]) - (y[5] - y[3]) + (y[7] - y[1]) - (y[4] - y[2]));
+    const T fjyze = e * ((y[6] - y[0]) + (y[5] - y[3]) + (y[7] - y[1]) + (y[4] - y[2]));
+    const T fjzxi = e * ((z[6] - z[0]) + (z[5] - z[3]) - (z[7] - z[1]) - (z[4] - z[2]));
+    const T fjzet = e * ((z[6] - z[0]) - (z[5] - z[3]) + (z[7] - z[1]) - (z[4] - z[2]));
+    const T fjzze = e * ((z[6] - z[0]) + (z[5] - z[3]) + (z[7] - z[1]) + (z[4] - z[2]));
+    const T cjxet = -(fjyxi * fjzze) + (fjzxi * fjyze);
+    const T cjyet = (fjxxi * fjzze) - (fjzxi * fjxze);
+    const T cjzet = -(fjxxi * fjyze) + (fjyxi * fjxze);
+    return T(8.) * (fjxet * cjxet + fjyet * cjyet + fjzet * cjzet);
 }
 
 // reference/C/LULESH-OMP.cc:620-706 in full: B[3][4] for nodes 0..3 (nodes
 // 4..7 are their negatives in the order 2,3,0,1) and the determinant
-__device__ __forceinline__ double shape_derivatives(const double *x,
-                                                   const double *y,
-                                                   const double *z,
-                                                   double b[3][4])
+template <typename T>
+__device__ __forceinline__ T shape_derivatives(const T *x, const T *y, const T *z,
+                                               T b[3][4])
 {
-    const double fjxxi = .125 * ((x[6] - x[0]) + (x[5] - x[3]) - (x[7] - x[1]) - (x[4] - x[2]));
-    const double fjxet = .125 * ((x[6] - x[0]) - (x[5] - x[3]) + (x[7] - x[1]) - (x[4] - x[2]));
-    const double fjxze = .125 * ((x[6] - x[0]) + (x[5] - x[3]) + (x[7] - x[1]) + (x[4] - x[2]));
-    const double fjyxi = .125 * ((y[6] - y[0]) + (y[5] - y[3]) - (y[7] - y[1]) - (y[4] - y[2]));
-    const double fjyet = .125 * ((y[6] - y[0]) - (y[5] - y[3]) + (y[7] - y[1]) - (y[4] - y[2]));
-    const double fjyze = .125 * ((y[6] - y[0]) + (y[5] - y[3]) + (y[7] - y[1]) + (y[4] - y[2]));
-    const double fjzxi = .125 * ((z[6] - z[0]) + (z[5] - z[3]) - (z[7] - z[1]) - (z[4] - z[2]));
-    const double fjzet = .125 * ((z[6] - z[0]) - (z[5] - z[3]) + (z[7] - z[1]) - (z[4] - z[2]));
-    const double fjzze = .125 * ((z[6] - z[0]) + (z[5] - z[3]) + (z[7] - z[1]) + (z[4] - z[2]));
+    const T e = T(.125);
+    const T fjxxi = e * ((x[6] - x[0]) + (x[5] - x[3]) - (x[7] - x[1]) - (x[4] - x[2]));
+    const T fjxet = e * ((x[6] - x[0]) - (x[5] - x[3]) + (x[7] - x[1]) - (x[4] - x[2]));
+    const T fjxze = e * ((x[6] - x[0]) + (x[5] - x[3]) + (x[7] - x[1]) + (x[4] - x[2]));
+    const T fjyxi = e * ((y[6] - y[0]) + (y[5] - y[3]) - (y[7] - y[1]) - (y[4] - y[2]));
+    const T fjyet = e * ((y[6] - y[0]) - (y[5] - y[3]) + (y[7] - y[1]) - (y[4] - y[2]));
+    const T fjyze = e * ((y[6] - y[0]) + (y[5] - y[3]) + (y[7] - y[1]) + (y[4] - y[2]));
+    const T fjzxi = e * ((z[6] - z[0]) + (z[5] - z[3]) - (z[7] - z[1]) - (z[4] - z[2]));
+    const T fjzet = e * ((z[6] - z[0]) - (z[5] - z[3]) + (z[7] - z[1]) - (z[4] - z[2]));
+    const T fjzze = e * ((z[6] - z[0]) + (z[5] - z[3]) + (z[7] - z[1]) + (z[4] - z[2]));
 
-    const double cjxxi = (fjyet * fjzze) - (fjzet * fjyze);
-    const double cjxet = -(fjyxi * fjzze) + (fjzxi * fjyze);
-    const double cjxze = (fjyxi * fjzet) - (fjzxi * fjyet);
-    const double cjyxi = -(fjxet * fjzze) + (fjzet * fjxze);
-    const double cjyet = (fjxxi * fjzze) - (fjzxi * fjxze);
-    const double cjyze = -(fjxxi * fjzet) + (fjzxi * fjxet);
-    const double cjzxi = (fjxet * fjyze) - (fjyet * fjxze);
-    const double cjzet = -(fjxxi * fjyze) + (fjyxi * fjxze);
-    const double cjzze = (fjxxi * fjyet) - (fjyxi * fjxet);
+    const T cjxxi = (fjyet * fjzze) - (fjzet * fjyze);
+    const T cjxet = -(fjyxi * fjzze) + (fjzxi * fjyze);
+    const T cjxze = (fjyxi * fjzet) - (fjzxi * fjyet);
+    const T cjyxi = -(fjxet * fjzze) + (fjzet * fjxze);
+    const T cjyet = (fjxxi * fjzze) - (fjzxi * fjxze);
+    const T cjyze = -(fjxxi * fjzet) + (fjzxi * fjxet);
+    const T cjzxi = (fjxet * fjyze) - (fjyet * fjxze);
+    const T cjzet = -(fjxxi * fjyze) + (fjyxi * fjxze);
+    const T cjzze = (fjxxi * fjyet) - (fjyxi * fjxet);
 
     b[0][0] = -cjxxi - cjxet - cjxze;
     b[0][1] = cjxxi - cjxet - cjxze;
@@ -83,39 +100,40 @@ __device__ __forceinline__ double shape_derivatives(const double *x,
     b[2][1] = cjzxi - cjzet - cjzze;
     b[2][2] = cjzxi + cjzet - cjzze;
     b[2][3] = -cjzxi + cjzet - cjzze;
-    return 8. * (fjxet * cjxet + fjyet * cjyet + fjzet * cjzet);
+    return T(8.) * (fjxet * cjxet + fjyet * cjyet + fjzet * cjzet);
 }
 
 // reference/C/LULESH-OMP.cc:709-742
-__device__ __forceinline__ void face_normal(double *pfx, double *pfy,
-                                            double *pfz, const double *x,
-                                            const double *y, const double *z,
-                                            int n0, int n1, int n2, int n3)
+template <typename T>
+__device__ __forceinline__ void face_normal(T *pfx, T *pfy, T *pfz, const T *x,
+                                            const T *y, const T *z, int n0, int n1,
+                                            int n2, int n3)
 {
-    const double bx0 = 0.5 * (x[n3] + x[n2] - x[n1] - x[n0]);
-    const double by0 = 0.5 * (y[n3] + y[n2] - y[n1] - y[n0]);
-    const double bz0 = 0.5 * (z[n3] + z[n2] - z[n1] - z[n0]);
-    const double bx1 = 0.5 * (x[n2] + x[n1] - x[n3] - x[n0]);
-    const double by1 = 0.5 * (y[n2] + y[n1] - y[n3] - y[n0]);
-    const double bz1 = 0.5 * (z[n2] + z[n1] - z[n3] - z[n0]);
-    const double ax = 0.25 * (by0 * bz1 - bz0 * by1);
-    const double ay = 0.25 * (bz0 * bx1 - bx0 * bz1);
-    const double az = 0.25 * (bx0 * by1 - by0 * bx1);
+    const T h = T(0.5), q = T(0.25);
+    const T bx0 = h * (x[n3] + x[n2] - x[n1] - x[n0]);
+    const T by0 = h * (y[n3] + y[n2] - y[n1] - y[n0]);
+    const T bz0 = h * (z[n3] + z[n2] - z[n1] - z[n0]);
+    const T bx1 = h * (x[n2] + x[n1] - x[n3] - x[n0]);
+    const T by1 = h * (y[n2] + y[n1] - y[n3] - y[n0]);
+    const T bz1 = h * (z[n2] + z[n1] - z[n3] - z[n0]);
+    const T ax = q * (by0 * bz1 - bz0 * by1);
+    const T ay = q * (bz0 * bx1 - bx0 * bz1);
+    const T az = q * (bx0 * by1 - by0 * bx1);
     pfx[n0] += ax; pfx[n1] += ax; pfx[n2] += ax; pfx[n3] += ax;
     pfy[n0] += ay; pfy[n1] += ay; pfy[n2] += ay; pfy[n3] += ay;
     pfz[n0] += az; pfz[n1] += az; pfz[n2] += az; pfz[n3] += az;
 }
 
 // reference/C/LULESH-OMP.cc:745-799
-__device__ __forceinline__ void node_normals(double *pfx, double *pfy,
-                                             double *pfz, const double *x,
-                                             const double *y, const double *z)
+template <typename T>
+__device__ __forceinline__ void node_normals(T *pfx, T *pfy, T *pfz, const T *x,
+                                             const T *y, const T *z)
 {
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        pfx[i] = 0.0;
-        pfy[i] = 0.0;
-        pfz[i] = 0.0;
+        pfx[i] = T(0.0);
+        pfy[i] = T(0.0);
+        pfz[i] = T(0.0);
     }
     face_normal(pfx, pfy, pfz, x, y, z, 0, 1, 2, 3);
     face_normal(pfx, pfy, pfz, x, y, z, 0, 4, 5, 1);
@@ -125,17 +143,17 @@ __device__ __forceinline__ void node_normals(double *pfx, double *pfy,
     face_normal(pfx, pfy, pfz, x, y, z, 4, 7, 6, 5);
 }
 
-// reference/C/LULESH-OMP.cc:979-1006
-__device__ __forceinline__ void volu_der(const double *x, const double *y,
-                                         const double *z, int a0, int a1,
-                                         int a2, int a3, int a4, int a5,
-                                         double &dvdx, double &dvdy,
-                                         double &dvdz)
+// reference/C/LULESH-OMP.cc:979-1006; the node-by-node argument lists of
+// CalcElemVolumeDerivative are :1008-1048
+template <typename T>
+__device__ __forceinline__ void volu_der(const T *x, const T *y, const T *z, int a0,
+                                         int a1, int a2, int a3, int a4, int a5,
+                                         T &dvdx, T &dvdy, T &dvdz)
 {
-    const double twelfth = 1.0 / 12.0;
-    const double x0 = x[a0], x1 = x[a1], x2 = x[a2], x3 = x[a3], x4 = x[a4], x5 = x[a5];
-    const double y0 = y[a0], y1 = y[a1], y2 = y[a2], y3 = y[a3], y4 = y[a4], y5 = y[a5];
-    const double z0 = z[a0], z1 = z[a1], z2 = z[a2], z3 = z[a3], z4 = z[a4], z5 = z[a5];
+    const T twelfth = T(1.0 / 12.0);
+    const T x0 = x[a0], x1 = x[a1], x2 = x[a2], x3 = x[a3], x4 = x[a4], x5 = x[a5];
+    const T y0 = y[a0], y1 = y[a1], y2 = y[a2], y3 = y[a3], y4 = y[a4], y5 = y[a5];
+    const T z0 = z[a0], z1 = z[a1], z2 = z[a2], z3 = z[a3], z4 = z[a4], z5 = z[a5];
     dvdx = (y1 + y2) * (z0 + z1) - (y0 + y1) * (z1 + z2) +
            (y0 + y4) * (z3 + z4) - (y3 + y4) * (z0 + z4) -
            (y2 + y5) * (z3 + z5) + (y3 + y5) * (z2 + z5);
@@ -150,25 +168,10 @@ __device__ __forceinline__ void volu_der(const double *x, const double *y,
     dvdz *= twelfth;
 }
 
-// reference/C/LULESH-OMP.cc:1008-1048
-__device__ __forceinline__ void volume_derivative(double *dvdx, double *dvdy,
-                                                  double *dvdz, const double *x,
-                                                  const double *y,
-                                                  const double *z)
-{
-    volu_der(x, y, z, 1, 2, 3, 4, 5, 7, dvdx[0], dvdy[0], dvdz[0]);
-    volu_der(x, y, z, 0, 1, 2, 7, 4, 6, dvdx[3], dvdy[3], dvdz[3]);
-    volu_der(x, y, z, 3, 0, 1, 6, 7, 5, dvdx[2], dvdy[2], dvdz[2]);
-    volu_der(x, y, z, 2, 3, 0, 5, 6, 4, dvdx[1], dvdy[1], dvdz[1]);
-    volu_der(x, y, z, 7, 6, 5, 0, 3, 1, dvdx[4], dvdy[4], dvdz[4]);
-    volu_der(x, y, z, 4, 7, 6, 1, 0, 2, dvdx[5], dvdy[5], dvdz[5]);
-    volu_der(x, y, z, 5, 4, 7, 2, 1, 3, dvdx[6], dvdy[6], dvdz[6]);
-    volu_der(x, y, z, 6, 5, 4, 3, 2, 0, dvdx[7], dvdy[7], dvdz[7]);
-}
-
 // Flanagan-Belytschko hourglass base vectors (reference/C/LULESH-OMP.cc:
 // 1252-1285), one row per mode, entries +-1
-__device__ __forceinline__ double fb_gamma(int mode, int node)
+template <typename T = double>
+__device__ __forceinline__ T fb_gamma(int mode, int node)
 {
     constexpr signed char g[4][8] = {
         {1, 1, -1, -1, -1, -1, 1, 1},
@@ -176,89 +179,35 @@ __device__ __forceinline__ double fb_gamma(int mode, int node)
         {1, -1, 1, -1, 1, -1, 1, -1},
         {-1, 1, -1, 1, 1, -1, 1, -1},
     };
-    return double(g[mode][node]);
-}
-
-// Hourglass force of one element: hourgam = Gamma - (dV/dx . hourmod)/V
-// (reference/C/LULESH-OMP.cc:1304-1358) and hgf = coef * hourgam (hourgam^T
-// u) per velocity component (:1050-1231).
-__device__ __forceinline__ void fb_hourglass(const double *x, const double *y,
-                                             const double *z, const double *dvdx,
-                                             const double *dvdy,
-                                             const double *dvdz,
-                                             const double *xd, const double *yd,
-                                             const double *zd, double determ,
-                                             double coefficient, double *hgfx,
-                                             double *hgfy, double *hgfz)
-{
-    double hourgam[8][4];
-    const double volinv = 1.0 / determ;
-#pragma unroll
-    for (int i1 = 0; i1 < 4; ++i1) {
-        double hmx = x[0] * fb_gamma(i1, 0);
-        double hmy = y[0] * fb_gamma(i1, 0);
-        double hmz = z[0] * fb_gamma(i1, 0);
-#pragma unroll
-        for (int n = 1; n < 8; ++n) {
-            hmx = hmx + x[n] * fb_gamma(i1, n);
-            hmy = hmy + y[n] * fb_gamma(i1, n);
-            hmz = hmz + z[n] * fb_gamma(i1, n);
-        }
-#pragma unroll
-        for (int n = 0; n < 8; ++n) {
-            hourgam[n][i1] = fb_gamma(i1, n) -
-                             volinv * (dvdx[n] * hmx + dvdy[n] * hmy + dvdz[n] * hmz);
-        }
-    }
-    const double *vel[3] = {xd, yd, zd};
-    double *out[3] = {hgfx, hgfy, hgfz};
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const double *u = vel[c];
-        double h[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            h[i] = hourgam[0][i] * u[0] + hourgam[1][i] * u[1] +
-                   hourgam[2][i] * u[2] + hourgam[3][i] * u[3] +
-                   hourgam[4][i] * u[4] + hourgam[5][i] * u[5] +
-                   hourgam[6][i] * u[6] + hourgam[7][i] * u[7];
-        }
-#pragma unroll
-        for (int n = 0; n < 8; ++n) {
-            out[c][n] = coefficient * (hourgam[n][0] * h[0] + hourgam[n][1] * h[1] +
-                                       hourgam[n][2] * h[2] + hourgam[n][3] * h[3]);
-        }
-    }
+    return T(g[mode][node]);
 }
 
 // reference/C/LULESH-OMP.cc:1706-1789
-__device__ __forceinline__ double triple_product(double x1, double y1,
-                                                 double z1, double x2,
-                                                 double y2, double z2,
-                                                 double x3, double y3,
-                                                 double z3)
+template <typename T>
+__device__ __forceinline__ T triple_product(T x1, T y1, T z1, T x2, T y2, T z2, T x3,
+                                            T y3, T z3)
 {
     return x1 * (y2 * z3 - z2 * y3) + x2 * (z1 * y3 - y1 * z3) +
            x3 * (y1 * z2 - z1 * y2);
 }
 
-__device__ __forceinline__ double elem_volume(const double *x, const double *y,
-                                              const double *z)
+template <typename T>
+__device__ __forceinline__ T elem_volume(const T *x, const T *y, const T *z)
 {
-    const double twelveth = 1.0 / 12.0;
-    const double dx61 = x[6] - x[1], dy61 = y[6] - y[1], dz61 = z[6] - z[1];
-    const double dx70 = x[7] - x[0], dy70 = y[7] - y[0], dz70 = z[7] - z[0];
-    const double dx63 = x[6] - x[3], dy63 = y[6] - y[3], dz63 = z[6] - z[3];
-    const double dx20 = x[2] - x[0], dy20 = y[2] - y[0], dz20 = z[2] - z[0];
-    const double dx50 = x[5] - x[0], dy50 = y[5] - y[0], dz50 = z[5] - z[0];
-    const double dx64 = x[6] - x[4], dy64 = y[6] - y[4], dz64 = z[6] - z[4];
-    const double dx31 = x[3] - x[1], dy31 = y[3] - y[1], dz31 = z[3] - z[1];
-    const double dx72 = x[7] - x[2], dy72 = y[7] - y[2], dz72 = z[7] - z[2];
-    const double dx43 = x[4] - x[3], dy43 = y[4] - y[3], dz43 = z[4] - z[3];
-    const double dx57 = x[5] - x[7], dy57 = y[5] - y[7], dz57 = z[5] - z[7];
-    const double dx14 = x[1] - x[4], dy14 = y[1] - y[4], dz14 = z[1] - z[4];
-    const double dx25 = x[2] - x[5], dy25 = y[2] - y[5], dz25 = z[2] - z[5];
-    double volume =
+    const T twelveth = T(1.0 / 12.0);
+    const T dx61 = x[6] - x[1], dy61 = y[6] - y[1], dz61 = z[6] - z[1];
+    const T dx70 = x[7] - x[0], dy70 = y[7] - y[0], dz70 = z[7] - z[0];
+    const T dx63 = x[6] - x[3], dy63 = y[6] - y[3], dz63 = z[6] - z[3];
+    const T dx20 = x[2] - x[0], dy20 = y[2] - y[0], dz20 = z[2] - z[0];
+    const T dx50 = x[5] - x[0], dy50 = y[5] - y[0], dz50 = z[5] - z[0];
+    const T dx64 = x[6] - x[4], dy64 = y[6] - y[4], dz64 = z[6] - z[4];
+    const T dx31 = x[3] - x[1], dy31 = y[3] - y[1], dz31 = z[3] - z[1];
+    const T dx72 = x[7] - x[2], dy72 = y[7] - y[2], dz72 = z[7] - z[2];
+    const T dx43 = x[4] - x[3], dy43 = y[4] - y[3], dz43 = z[4] - z[3];
+    const T dx57 = x[5] - x[7], dy57 = y[5] - y[7], dz57 = z[5] - z[7];
+    const T dx14 = x[1] - x[4], dy14 = y[1] - y[4], dz14 = z[1] - z[4];
+    const T dx25 = x[2] - x[5], dy25 = y[2] - y[5], dz25 = z[2] - z[5];
+    T volume =
         triple_product(dx31 + dx72, dx63, dx20, dy31 + dy72, dy63, dy20,
                        dz31 + dz72, dz63, dz20) +
         triple_product(dx43 + dx57, dx64, dx70, dy43 + dy57, dy64, dy70,
@@ -270,40 +219,40 @@ __device__ __forceinline__ double elem_volume(const double *x, const double *y,
 }
 
 // reference/C/LULESH-OMP.cc:1799-1819
-__device__ __forceinline__ double area_face(const double *x, const double *y,
-                                            const double *z, int n0, int n1,
-                                            int n2, int n3)
+template <typename T>
+__device__ __forceinline__ T area_face(const T *x, const T *y, const T *z, int n0,
+                                       int n1, int n2, int n3)
 {
-    const double fx = (x[n2] - x[n0]) - (x[n3] - x[n1]);
-    const double fy = (y[n2] - y[n0]) - (y[n3] - y[n1]);
-    const double fz = (z[n2] - z[n0]) - (z[n3] - z[n1]);
-    const double gx = (x[n2] - x[n0]) + (x[n3] - x[n1]);
-    const double gy = (y[n2] - y[n0]) + (y[n3] - y[n1]);
-    const double gz = (z[n2] - z[n0]) + (z[n3] - z[n1]);
+    const T fx = (x[n2] - x[n0]) - (x[n3] - x[n1]);
+    const T fy = (y[n2] - y[n0]) - (y[n3] - y[n1]);
+    const T fz = (z[n2] - z[n0]) - (z[n3] - z[n1]);
+    const T gx = (x[n2] - x[n0]) + (x[n3] - x[n1]);
+    const T gy = (y[n2] - y[n0]) + (y[n3] - y[n1]);
+    const T gz = (z[n2] - z[n0]) + (z[n3] - z[n1]);
     return (fx * fx + fy * fy + fz * fz) * (gx * gx + gy * gy + gz * gz) -
            (fx * gx + fy * gy + fz * gz) * (fx * gx + fy * gy + fz * gz);
 }
 
 // std::max(a, b) as the reference's libstdc++ evaluates it
-__device__ __forceinline__ double ref_max(double a, double b)
+template <typename T>
+__device__ __forceinline__ T ref_max(T a, T b)
 {
     return (a < b) ? b : a;
 }
 
 // reference/C/LULESH-OMP.cc:1821-1862
-__device__ __forceinline__ double characteristic_length(const double *x,
-                                                        const double *y,
-                                                        const double *z,
-                                                        double volume)
+template <typename T>
+__device__ __forceinline__ T characteristic_length(const T *x, const T *y, const T *z,
+                                                   T volume)
 {
-    double cl = 0.0;
+    T cl = T(0.0);
     cl = ref_max(area_face(x, y, z, 0, 1, 2, 3), cl);
     cl = ref_max(area_face(x, y, z, 4, 5, 6, 7), cl);
     cl = ref_max(area_face(x, y, z, 0, 1, 5, 4), cl);
     cl = ref_max(area_face(x, y, z, 1, 2, 6, 5), cl);
     cl = ref_max(area_face(x, y, z, 2, 3, 7, 6), cl);
     cl = ref_max(area_face(x, y, z, 3, 0, 4, 7), cl);
-    return 4.0 * volume / sqrt(cl);
+    return T(4.0) * volume / elem_sqrt(cl);
 }
 
 // Equation of state of one element: CalcEnergyForElems with its three

@@ -92,6 +92,23 @@ struct DevFields {
     const uint16_t *eos_rep;
 };
 
+// The mixed-precision domain: the node arrays, the energy, the volumes and
+// the masses stay float64; the element-local quantities of the stress,
+// hourglass, Q and EOS arithmetic (pressure, q and its linear / quadratic
+// parts, sound speed, the kinematic rates and the Q gradients) are float32.
+struct DevFieldsMixed {
+    int nx;
+    int num_elem, num_node;
+    FastDiv div_nx;
+    double *x[2], *y[2], *z[2], *xd[2], *yd[2], *zd[2];
+    double *nodal_mass;
+    double *e, *v, *volo, *elem_mass, *vnew;
+    float *p, *q, *ql, *qq, *ss, *delv, *arealg, *vdov;
+    float *delv_xi, *delv_eta, *delv_zeta, *delx_xi, *delx_eta, *delx_zeta;
+    const int32_t *eos_order;
+    const uint16_t *eos_rep;
+};
+
 // Launch shape of the force sweep: TX x TY element columns per block (one
 // thread each), NZ node planes per block, and the number of waves per SIMD
 // the kernel's register budget must leave room for.
@@ -147,5 +164,13 @@ hipError_t launch_force_sweep_batch(const SweepShape &shape, int blocks,
                                     const BatchMember *members, const BatchWork *work,
                                     const PhysParams &prm, hipStream_t stream);
 bool force_sweep_shape_supported(const SweepShape &shape);
+
+// The mixed-precision phases (the time step kernels are shared).
+__global__ void k_kinematics_q_mixed(DevFieldsMixed f, DevScalars *s, PhysParams prm);
+__global__ void k_q_eos_dt_mixed(DevFieldsMixed f, DevScalars *s, PhysParams prm);
+hipError_t launch_force_sweep_mixed(const SweepShape &shape, const DevFieldsMixed &f,
+                                    DevScalars *s, const PhysParams &prm,
+                                    hipStream_t stream);
+bool force_sweep_mixed_shape_supported(const SweepShape &shape);
 
 }  // namespace lulesh

@@ -13,6 +13,8 @@ namespace {
 size_t round_up(size_t n, size_t m) { return (n + m - 1) / m * m; }
 
 constexpr size_t kNodeArrays = 13, kElemArrays = 19;
+// mixed precision: 5 float64 and 14 float32 element arrays
+constexpr size_t kElemArrays64 = 5, kElemArrays32 = 14;
 
 }  // namespace
 
@@ -134,6 +136,109 @@ DevScalars initial_scalars(const HostMesh &m, double stoptime, int max_cycles)
     s.active = 0;
     s.error = 0;
     return s;
+}
+
+size_t domain_arena_bytes_mixed(const HostMesh &m)
+{
+    const size_t pe = round_up(size_t(m.num_elem), 64);
+    const size_t pn = round_up(size_t(m.num_node), 64);
+    size_t bytes = (kNodeArrays * pn + kElemArrays64 * pe) * sizeof(double) +
+                   kElemArrays32 * pe * sizeof(float);
+    bytes += round_up(pe * sizeof(int32_t), 256) + round_up(pe * sizeof(uint16_t), 256);
+    bytes += round_up(sizeof(DevScalars), 256);
+    return bytes;
+}
+
+void carve_domain_mixed(char *base, const HostMesh &m, DevFieldsMixed &f, DevScalars **s)
+{
+    const size_t pe = round_up(size_t(m.num_elem), 64);
+    const size_t pn = round_up(size_t(m.num_node), 64);
+    double *cur = reinterpret_cast<double *>(base);
+    auto take_node = [&]() { double *p = cur; cur += pn; return p; };
+    auto take_elem = [&]() { double *p = cur; cur += pe; return p; };
+    f.nx = m.nx;
+    f.num_elem = int(m.num_elem);
+    f.num_node = int(m.num_node);
+    f.div_nx = make_fastdiv(unsigned(m.nx));
+    for (int c = 0; c < 2; ++c) {
+        f.x[c] = take_node(); f.y[c] = take_node(); f.z[c] = take_node();
+        f.xd[c] = take_node(); f.yd[c] = take_node(); f.zd[c] = take_node();
+    }
+    f.nodal_mass = take_node();
+    f.e = take_elem(); f.v = take_elem(); f.volo = take_elem();
+    f.elem_mass = take_elem(); f.vnew = take_elem();
+    float *cf = reinterpret_cast<float *>(cur);
+    auto take_f32 = [&]() { float *p = cf; cf += pe; return p; };
+    f.p = take_f32(); f.q = take_f32(); f.ql = take_f32(); f.qq = take_f32();
+    f.ss = take_f32(); f.delv = take_f32(); f.arealg = take_f32(); f.vdov = take_f32();
+    f.delv_xi = take_f32(); f.delv_eta = take_f32(); f.delv_zeta = take_f32();
+    f.delx_xi = take_f32(); f.delx_eta = take_f32(); f.delx_zeta = take_f32();
+    char *sched = reinterpret_cast<char *>(cf);
+    f.eos_order = reinterpret_cast<int32_t *>(sched);
+    sched += round_up(pe * sizeof(int32_t), 256);
+    f.eos_rep = reinterpret_cast<uint16_t *>(sched);
+    sched += round_up(pe * sizeof(uint16_t), 256);
+    *s = reinterpret_cast<DevScalars *>(sched);
+}
+
+hipError_t upload_mesh_mixed(const HostMesh &m, const DevFieldsMixed &f,
+                             hipStream_t stream)
+{
+    // the float64 arrays the mesh initialises have the same places in a
+    // DevFields view
+    DevFields v{};
+    for (int c = 0; c < 2; ++c) {
+        v.x[c] = f.x[c]; v.y[c] = f.y[c]; v.z[c] = f.z[c];
+    }
+    v.nodal_mass = f.nodal_mass;
+    v.e = f.e;
+    v.volo = f.volo;
+    v.elem_mass = f.elem_mass;
+    v.v = f.v;
+    v.eos_order = f.eos_order;
+    v.eos_rep = f.eos_rep;
+    return upload_mesh(m, v, stream);
+}
+
+void *field_pointer_mixed(const DevFieldsMixed &f, int id, int par, int64_t *count,
+                          bool *f32)
+{
+    const int64_t ne = f.num_elem, nn = f.num_node;
+    void *p = nullptr;
+    int64_t n = ne;
+    bool single = true;
+    switch (id) {
+    case LULESH_F_X: p = f.x[par]; n = nn; single = false; break;
+    case LULESH_F_Y: p = f.y[par]; n = nn; single = false; break;
+    case LULESH_F_Z: p = f.z[par]; n = nn; single = false; break;
+    case LULESH_F_XD: p = f.xd[par]; n = nn; single = false; break;
+    case LULESH_F_YD: p = f.yd[par]; n = nn; single = false; break;
+    case LULESH_F_ZD: p = f.zd[par]; n = nn; single = false; break;
+    case LULESH_F_NODAL_MASS: p = f.nodal_mass; n = nn; single = false; break;
+    case LULESH_F_E: p = f.e; single = false; break;
+    case LULESH_F_V: p = f.v; single = false; break;
+    case LULESH_F_VOLO: p = f.volo; single = false; break;
+    case LULESH_F_ELEM_MASS: p = f.elem_mass; single = false; break;
+    case LULESH_F_VNEW: p = f.vnew; single = false; break;
+    case LULESH_F_P: p = f.p; break;
+    case LULESH_F_Q: p = f.q; break;
+    case LULESH_F_QL: p = f.ql; break;
+    case LULESH_F_QQ: p = f.qq; break;
+    case LULESH_F_DELV: p = f.delv; break;
+    case LULESH_F_VDOV: p = f.vdov; break;
+    case LULESH_F_AREALG: p = f.arealg; break;
+    case LULESH_F_SS: p = f.ss; break;
+    case LULESH_F_DELV_XI: p = f.delv_xi; break;
+    case LULESH_F_DELV_ETA: p = f.delv_eta; break;
+    case LULESH_F_DELV_ZETA: p = f.delv_zeta; break;
+    case LULESH_F_DELX_XI: p = f.delx_xi; break;
+    case LULESH_F_DELX_ETA: p = f.delx_eta; break;
+    case LULESH_F_DELX_ZETA: p = f.delx_zeta; break;
+    default: n = 0; break;
+    }
+    *count = n;
+    *f32 = single;
+    return p;
 }
 
 double *field_pointer(const DevFields &f, int id, int par, int64_t *count)

@@ -34,6 +34,16 @@ hipError_t upload_mesh(const HostMesh &m, const DevFields &f, hipStream_t stream
 // the node arrays holding the state); nullptr for an unknown id.
 double *field_pointer(const DevFields &f, int id, int par, int64_t *count);
 
+// The same for the mixed-precision domain (float32 element-local arrays);
+// field_pointer_mixed sets *f32 when the field is stored as float32.
+size_t domain_arena_bytes_mixed(const HostMesh &m);
+void carve_domain_mixed(char *base, const HostMesh &m, DevFieldsMixed &f,
+                        DevScalars **s);
+hipError_t upload_mesh_mixed(const HostMesh &m, const DevFieldsMixed &f,
+                             hipStream_t stream);
+void *field_pointer_mixed(const DevFieldsMixed &f, int id, int par, int64_t *count,
+                          bool *f32);
+
 // The loop scalars before the first cycle.
 DevScalars initial_scalars(const HostMesh &m, double stoptime, int max_cycles);
 

@@ -60,7 +60,8 @@ _lib = None
 
 # every symbol declared in include/lulesh_mi355x.h
 EXPORTS = (
-    "lulesh_config_default", "lulesh_create", "lulesh_destroy", "lulesh_step",
+    "lulesh_config_default", "lulesh_create", "lulesh_destroy",
+    "lulesh_create_mixed", "lulesh_is_mixed", "lulesh_step",
     "lulesh_run", "lulesh_upload_state", "lulesh_download_state",
     "lulesh_get_field", "lulesh_set_field", "lulesh_get_stats",
     "lulesh_set_scalars", "lulesh_set_parameter", "lulesh_set_sweep_shape",
@@ -74,6 +75,9 @@ EXPORTS = (
     "lulesh_batch_set_shape", "lulesh_batch_get_shape",
     "lulesh_batch_autotune", "lulesh_batch_last_ms",
     "lulesh_batch_last_error",
+    "lulesh_mlp_param_count", "lulesh_mlp_create", "lulesh_mlp_destroy",
+    "lulesh_mlp_get_params", "lulesh_mlp_eval", "lulesh_mlp_gradient",
+    "lulesh_mlp_train", "lulesh_mlp_last_error",
 )
 
 
@@ -96,6 +100,10 @@ def lib() -> ctypes.CDLL:
     L.lulesh_create.restype = I
     L.lulesh_destroy.argtypes = [P]
     L.lulesh_destroy.restype = None
+    L.lulesh_create_mixed.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(P)]
+    L.lulesh_create_mixed.restype = I
+    L.lulesh_is_mixed.argtypes = [P]
+    L.lulesh_is_mixed.restype = I
     L.lulesh_step.argtypes = [P, I, IP]
     L.lulesh_step.restype = I
     L.lulesh_run.argtypes = [P, IP]
@@ -163,6 +171,24 @@ def lib() -> ctypes.CDLL:
     L.lulesh_batch_last_ms.restype = D
     L.lulesh_batch_last_error.argtypes = [P]
     L.lulesh_batch_last_error.restype = ctypes.c_char_p
+    FP = ctypes.POINTER(ctypes.c_float)
+    L.lulesh_mlp_param_count.argtypes = []
+    L.lulesh_mlp_param_count.restype = I
+    L.lulesh_mlp_create.argtypes = [FP, FP, FP, ctypes.POINTER(P)]
+    L.lulesh_mlp_create.restype = I
+    L.lulesh_mlp_destroy.argtypes = [P]
+    L.lulesh_mlp_destroy.restype = None
+    L.lulesh_mlp_get_params.argtypes = [P, FP]
+    L.lulesh_mlp_get_params.restype = I
+    L.lulesh_mlp_eval.argtypes = [P, FP, I, FP]
+    L.lulesh_mlp_eval.restype = I
+    L.lulesh_mlp_gradient.argtypes = [P, FP, FP, I, FP, FP, DP]
+    L.lulesh_mlp_gradient.restype = I
+    L.lulesh_mlp_train.argtypes = [P, FP, FP, I, FP, I, I, ctypes.c_float,
+                                   ctypes.c_ulonglong, DP]
+    L.lulesh_mlp_train.restype = I
+    L.lulesh_mlp_last_error.argtypes = [P]
+    L.lulesh_mlp_last_error.restype = ctypes.c_char_p
     _lib = L
     return L
 
@@ -182,18 +208,21 @@ class Domain:
 
     def __init__(self, nx: int = 30, ic_mode: int = 2, num_reg: int = 11,
                  balance: int = 1, cost: int = 1, max_cycles: int = 9999999,
-                 stoptime: float = 1.0e-2):
+                 stoptime: float = 1.0e-2, mixed: bool = False):
+        """mixed=True builds the mixed-precision form (lulesh_create_mixed)."""
         L = lib()
         cfg = default_config()
         cfg.nx, cfg.ic_mode, cfg.num_reg = nx, ic_mode, num_reg
         cfg.balance, cfg.cost = balance, cost
         cfg.max_cycles, cfg.stoptime = max_cycles, stoptime
         h = ctypes.c_void_p()
-        rc = L.lulesh_create(ctypes.byref(cfg), ctypes.byref(h))
+        create = L.lulesh_create_mixed if mixed else L.lulesh_create
+        rc = create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != OK:
             raise LuleshError(rc, "lulesh_create failed")
         self.h = h
         self.nx, self.ic_mode = nx, ic_mode
+        self.mixed = bool(L.lulesh_is_mixed(h))
         st = self.stats()
         self.num_elem, self.num_node = st.num_elem, st.num_node
 
@@ -311,6 +340,33 @@ class Domain:
         self._check(lib().lulesh_upload_state(
             self.h, *[_dp(b) for b in bufs], float(st["time"]),
             float(st["deltatime"])))
+
+    # -- integral diagnostics -------------------------------------------
+    def total_energy(self) -> float:
+        """Internal plus kinetic energy: sum(e * elemMass) +
+        1/2 sum(nodalMass * |xd|^2)."""
+        ie = float(np.dot(self.get("e"), self.get("elemMass")))
+        v2 = self.get("xd") ** 2 + self.get("yd") ** 2 + self.get("zd") ** 2
+        return ie + 0.5 * float(np.dot(self.get("nodalMass"), v2))
+
+    def shock_radius(self) -> dict:
+        """Distance from the origin of the centroid of the element with the
+        largest pressure, over the whole mesh and along the x axis."""
+        n, en = self.nx, self.nx + 1
+        p = self.get("p").reshape(n, n, n)          # [k, j, i]
+        x = self.get("x").reshape(en, en, en)
+        y = self.get("y").reshape(en, en, en)
+        z = self.get("z").reshape(en, en, en)
+
+        def centroid(a):
+            return 0.125 * (a[:-1, :-1, :-1] + a[:-1, :-1, 1:] + a[:-1, 1:, :-1] +
+                            a[:-1, 1:, 1:] + a[1:, :-1, :-1] + a[1:, :-1, 1:] +
+                            a[1:, 1:, :-1] + a[1:, 1:, 1:])
+        r = np.sqrt(centroid(x) ** 2 + centroid(y) ** 2 + centroid(z) ** 2)
+        k = int(np.argmax(p))
+        ix = int(np.argmax(p[0, 0, :]))
+        return {"radius": float(r.reshape(-1)[k]),
+                "radius_x_axis": float(r[0, 0, ix])}
 
     # -- timing ---------------------------------------------------------
     def set_kernel_timing(self, enable: bool) -> None:
@@ -476,3 +532,76 @@ class Batch:
             _, st = self.stats(i)
             tot += int(st.num_elem) * int(st.cycle)
         return tot
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class Mlp:
+    """The learned-Q network on the GPU (12-64-64-2, bf16 MFMA, fp32
+    master weights)."""
+
+    N_IN, N_OUT = 12, 2
+
+    def __init__(self, params, in_mean, in_std):
+        L = lib()
+        self.n_params = L.lulesh_mlp_param_count()
+        p = np.ascontiguousarray(params, np.float32)
+        m = np.ascontiguousarray(in_mean, np.float32)
+        sd = np.ascontiguousarray(in_std, np.float32)
+        if p.size != self.n_params or m.size != 12 or sd.size != 12:
+            raise ValueError("parameter or normalisation size mismatch")
+        h = ctypes.c_void_p()
+        rc = L.lulesh_mlp_create(_fp(p), _fp(m), _fp(sd), ctypes.byref(h))
+        if rc != OK:
+            raise LuleshError(rc, "lulesh_mlp_create failed")
+        self.h = h
+        self.in_mean, self.in_std = m.copy(), sd.copy()
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().lulesh_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int) -> None:
+        if rc != OK:
+            raise LuleshError(rc, lib().lulesh_mlp_last_error(self.h).decode())
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.n_params, np.float32)
+        self._check(lib().lulesh_mlp_get_params(self.h, _fp(out)))
+        return out
+
+    def eval(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 12)
+        y = np.empty((len(x), 2), np.float32)
+        self._check(lib().lulesh_mlp_eval(self.h, _fp(x), len(x), _fp(y)))
+        return y
+
+    def gradient(self, x, t, out_weight=(1.0, 1.0)) -> tuple[np.ndarray, float]:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 12)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1, 2)
+        w = np.ascontiguousarray(out_weight, np.float32)
+        g = np.empty(self.n_params, np.float32)
+        loss = ctypes.c_double()
+        self._check(lib().lulesh_mlp_gradient(self.h, _fp(x), _fp(t), len(x), _fp(w),
+                                              _fp(g), ctypes.byref(loss)))
+        return g, loss.value
+
+    def train(self, x, t, epochs: int, batch: int = 1024, lr: float = 1e-3,
+              seed: int = 0, out_weight=(1.0, 1.0)) -> float:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 12)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1, 2)
+        w = np.ascontiguousarray(out_weight, np.float32)
+        loss = ctypes.c_double()
+        self._check(lib().lulesh_mlp_train(self.h, _fp(x), _fp(t), len(x), _fp(w),
+                                           int(epochs), int(batch), float(lr), int(seed),
+                                           ctypes.byref(loss)))
+        return loss.value
