@@ -15,8 +15,8 @@ host's cores on a 20-cycle sample of the same workload.
 
 The "forms" object holds the other forms of the step on their own
 workloads: the ensemble (64 Sedov members of uneven size, each to t=1e-2)
-and the mixed-precision step (120^3 to t=1e-2, with its errors against the
-float64 run).
+the mixed-precision step (120^3 to t=1e-2, with its errors against the
+float64 run) and the learned-Q closure (40^3 against the 120^3 solution).
 
     python bench.py [--steps K] [--warmup W] [--nx N] [--no-cpu-baseline]
                     [--no-forms]
@@ -204,6 +204,30 @@ def bench_mixed(lulesh_amd, nx: int = 120) -> dict:
     }
 
 
+def bench_learned_q(lulesh_amd, efactor: float = 1.0) -> dict:
+    """The learned-Q form: the network shipped with the package
+    (lulesh-accelerate_amd/learned_q_net.npz, trained by learned_q.py on
+    fine 120^3 runs of other energy factors) at 40^3, beside the plain
+    limiter at 40^3, both judged by the 120^3 run of the same factor
+    restricted to 40^3 at t = 1e-2."""
+    import learned_q
+    net, meta = learned_q.load()
+    res = learned_q.evaluate(net, [efactor])[f"{efactor:.4f}"]
+    plain, learned = res["plain"], res["learned"]
+    return {
+        "value": learned["zones_cycles_per_s"], "unit": "zones·cycles/s",
+        "workload": "sedov 40^3 (LULESH 2.0 ICs, 11 regions) to t=1e-2, learned "
+                    "closure vs plain limiter, errors against the 120^3 run "
+                    "restricted to 40^3",
+        "plain_40": plain, "learned_40": learned,
+        "fine_120_cycles": res["fine_cycles"],
+        "error_ratio_learned_over_plain": {
+            k: learned[k] / plain[k] for k in ("density_l1", "energy_l1", "pressure_l1")},
+        "training": {k: meta[k] for k in ("samples", "epochs", "loss_init", "loss_final",
+                                          "loss_of_plain_limiter")},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -286,6 +310,7 @@ def main() -> int:
                            "mixed": bench_mixed(lulesh_amd)}
         result["forms"]["mixed"]["vs_plain_150"] = (
             result["forms"]["mixed"]["value"] / value)
+        result["forms"]["learned_q"] = bench_learned_q(lulesh_amd)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(nx, args.cpu_cycles)
     print(json.dumps(result))

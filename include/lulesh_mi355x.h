@@ -100,6 +100,11 @@ void lulesh_destroy(lulesh_domain *d);
  * way in.  Its results differ from lulesh_create's by rounding: the origin
  * energy at t = 1e-2 within 1e-5 relative at 120^3. */
 int lulesh_create_mixed(const lulesh_config *cfg, lulesh_domain **out);
+/* lulesh_create (mixed = 0) or lulesh_create_mixed (mixed = 1) with the
+ * initial origin energy multiplied by efactor (and, in 2.0 mode, the initial
+ * dt derived from it): one member of a seeded ensemble of Sedov problems. */
+int lulesh_create_scaled(const lulesh_config *cfg, double efactor, int mixed,
+                         lulesh_domain **out);
 /* 1 for a domain made by lulesh_create_mixed, else 0. */
 int lulesh_is_mixed(const lulesh_domain *d);
 
@@ -279,6 +284,24 @@ int lulesh_mlp_train(lulesh_mlp *n, const float *x, const float *t, int count,
                      const float *out_weight, int epochs, int batch, float lr,
                      unsigned long long seed, double *final_loss);
 const char *lulesh_mlp_last_error(const lulesh_mlp *n);
+
+/* The learned closure in a domain's Q phase: with net != NULL the linear
+ * and quadratic Q of the monotonic limiter (reference/C/LULESH-OMP.cc:
+ * 2182-2320) of each compressing element are multiplied by the network's
+ * two outputs m_l, m_q (clamped to [0, 3]; 1 is the plain limiter); an
+ * element keeps the plain limiter when an input is more than z_max standard
+ * deviations from the training mean or its volume / energy is not
+ * positive.  net == NULL restores the plain limiter.  The domain uses the
+ * network until it is reset: the network must outlive that use.  Float64
+ * domains only. */
+int lulesh_set_learned_q(lulesh_domain *d, const lulesh_mlp *net, double z_max);
+/* The network inputs of every element of the current state (features
+ * [num_elem][12], in the order of lulesh_kernels.hip) after forming its
+ * kinematics and Q gradients, and q[num_elem][4] = (-rho sum du,
+ * rho sum du^2, plain ql, plain qq), the first two 0 for an element without
+ * compression.  Overwrites vnew, delv, arealg, vdov and the Q gradients;
+ * returns the error code of a non-positive volume, if any. */
+int lulesh_q_features(lulesh_domain *d, float *features, double *q);
 
 #ifdef __cplusplus
 }

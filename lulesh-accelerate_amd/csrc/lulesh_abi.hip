@@ -16,6 +16,7 @@
 
 #include "lulesh_kernels.hpp"
 #include "lulesh_mesh.hpp"
+#include "lulesh_mlp.hpp"
 #include "lulesh_setup.hpp"
 
 using namespace lulesh;
@@ -62,6 +63,8 @@ struct lulesh_domain {
     std::vector<int32_t> reg_num_list;
 
     bool mixed = false;   // float32 element-local arithmetic (fm), else f
+    const MlpFrags *learned = nullptr;  // learned-Q network, if set
+    float learned_z_max = 0.f;
     DevFields f{};
     DevFieldsMixed fm{};
     DevScalars *s = nullptr;
@@ -170,6 +173,9 @@ int queue_cycle(lulesh_domain *d)
              if (mixed) {
                  hipLaunchKernelGGL(k_q_eos_dt_mixed, dim3(ge), dim3(kBlock), 0, st, fm, s,
                                     p);
+             } else if (d->learned) {
+                 hipLaunchKernelGGL(k_q_eos_dt_learned, dim3(ge), dim3(kBlock), 0, st, f, s,
+                                    p, d->learned, d->learned_z_max);
              } else {
                  hipLaunchKernelGGL(k_q_eos_dt, dim3(ge), dim3(kBlock), 0, st, f, s, p);
              }
@@ -308,7 +314,8 @@ void lulesh_config_default(lulesh_config *cfg)
 
 namespace {
 
-int create_domain(const lulesh_config *cfg, bool mixed, lulesh_domain **out)
+int create_domain(const lulesh_config *cfg, bool mixed, double efactor,
+                  lulesh_domain **out)
 {
     if (!cfg || !out) {
         return LULESH_BAD_ARG;
@@ -332,6 +339,7 @@ int create_domain(const lulesh_config *cfg, bool mixed, lulesh_domain **out)
         mc.num_reg = cfg->num_reg;
         mc.balance = cfg->balance;
         mc.cost = cfg->cost;
+        mc.efactor = efactor;
         m = build_mesh(mc);
     } catch (const std::exception &ex) {
         delete d;
@@ -369,12 +377,18 @@ extern "C" {
 
 int lulesh_create(const lulesh_config *cfg, lulesh_domain **out)
 {
-    return create_domain(cfg, false, out);
+    return create_domain(cfg, false, 1.0, out);
 }
 
 int lulesh_create_mixed(const lulesh_config *cfg, lulesh_domain **out)
 {
-    return create_domain(cfg, true, out);
+    return create_domain(cfg, true, 1.0, out);
+}
+
+int lulesh_create_scaled(const lulesh_config *cfg, double efactor, int mixed,
+                         lulesh_domain **out)
+{
+    return create_domain(cfg, mixed != 0, efactor, out);
 }
 
 int lulesh_is_mixed(const lulesh_domain *d) { return (d && d->mixed) ? 1 : 0; }
@@ -709,6 +723,70 @@ int lulesh_kernel_times(lulesh_domain *d, double *ms, long long *launches)
 }
 
 double lulesh_last_step_ms(lulesh_domain *d) { return d ? d->last_step_ms : 0.0; }
+
+int lulesh_set_learned_q(lulesh_domain *d, const lulesh_mlp *net, double z_max)
+{
+    if (!d || (net && !(z_max > 0.0))) {
+        return LULESH_BAD_ARG;
+    }
+    if (net && d->mixed) {
+        return fail(d, LULESH_BAD_ARG, "the learned closure runs on float64 domains");
+    }
+    d->learned = net ? lulesh_mlp_device_frags(net) : nullptr;
+    d->learned_z_max = float(z_max);
+    return LULESH_OK;
+}
+
+int lulesh_q_features(lulesh_domain *d, float *features, double *q)
+{
+    if (!d || !features || !q) {
+        return LULESH_BAD_ARG;
+    }
+    if (d->mixed) {
+        return fail(d, LULESH_BAD_ARG, "features are formed on float64 domains");
+    }
+    // kinematics and gradients of the current state (dt = 0: the half-step
+    // geometry is the current one) under a temporarily active cycle
+    DevScalars saved{};
+    int rc = read_scalars(d, &saved);
+    if (rc) return rc;
+    DevScalars tmp = saved;
+    tmp.active = 1;
+    tmp.error = 0;
+    tmp.deltatime = 0.0;
+    if ((rc = write_scalars(d, tmp))) return rc;
+    const unsigned ge = unsigned((d->num_elem + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_kinematics_q, dim3(ge), dim3(kBlock), 0, d->stream, d->f, d->s,
+                       d->prm);
+    HIP_TRY(d, hipGetLastError());
+    float *dfeat = nullptr;
+    double *dq = nullptr;
+    const size_t ne = size_t(d->num_elem);
+    HIP_TRY(d, hipMalloc(&dfeat, ne * kMlpIn * sizeof(float)));
+    if (hipMalloc(&dq, ne * 4 * sizeof(double)) != hipSuccess) {
+        (void)hipFree(dfeat);
+        return fail(d, LULESH_HIP_ERROR, "hipMalloc failed");
+    }
+    hipLaunchKernelGGL(k_q_features, dim3(ge), dim3(kBlock), 0, d->stream, d->f, d->prm,
+                       dfeat, dq);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(features, dfeat, ne * kMlpIn * sizeof(float),
+                           hipMemcpyDeviceToHost, d->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(q, dq, ne * 4 * sizeof(double), hipMemcpyDeviceToHost, d->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+    (void)hipFree(dfeat);
+    (void)hipFree(dq);
+    if (e != hipSuccess) {
+        return fail(d, LULESH_HIP_ERROR, std::string("q features: ") + hipGetErrorString(e));
+    }
+    DevScalars after{};
+    if ((rc = read_scalars(d, &after))) return rc;
+    saved.error = after.error;  // a non-positive volume is reported
+    if ((rc = write_scalars(d, saved))) return rc;
+    return after.error;
+}
 
 const char *lulesh_last_error(const lulesh_domain *d)
 {

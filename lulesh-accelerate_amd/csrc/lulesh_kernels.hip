@@ -21,6 +21,7 @@
 #include "lulesh_kernels.hpp"
 
 #include "lulesh_elem.hpp"
+#include "lulesh_mlp.hpp"
 
 #pragma clang fp contract(off)
 
@@ -364,52 +365,77 @@ __device__ __forceinline__ unsigned long long block_min(unsigned long long v,
 // ---------------------------------------------------------------------------
 namespace {
 
-// one block of 256 elements in schedule order; every thread of the block
-// calls it (the minima are reduced over the block)
-__device__ __forceinline__ void q_eos_dt_block(const DevFields &f, DevScalars *s,
-                                               const PhysParams &prm, int slot,
-                                               unsigned long long *lds_c,
-                                               unsigned long long *lds_h)
-{
-    unsigned long long cand_c = as_bits(kBig), cand_h = as_bits(kBig);
-    if (slot < f.num_elem) {
-        const int el = f.eos_order[slot];
-        const int rep = f.eos_rep[slot];
-        const int nx = f.nx;
-        int i, j, k;
-        elem_ijk(f, el, i, j, k);
-        const int sy = nx, sz = nx * nx;
+// What the monotonic-Q limiter of an element works from
+// (reference/C/LULESH-OMP.cc:2182-2320): the limiter values of the three
+// directions and, for a compressing element (vdov <= 0), the velocity
+// jumps across it (delv * delx, clamped to <= 0) and its density.
+struct QInputs {
+    double phi[3];
+    double du[3];
+    double rho;
+    double vdov, vnew, volo;
+};
 
-        // monotonic Q limiter and ql / qq
-        const double dvxi = f.delv_xi[el], dveta = f.delv_eta[el], dvzeta = f.delv_zeta[el];
-        const double lm = prm.monoq_limiter_mult, ms = prm.monoq_max_slope;
-        const double phixi = mono_phi(dvxi, (i == 0) ? dvxi : f.delv_xi[el - 1],
-                                      (i == nx - 1) ? 0.0 : f.delv_xi[el + 1], lm, ms);
-        const double phieta = mono_phi(dveta, (j == 0) ? dveta : f.delv_eta[el - sy],
-                                       (j == nx - 1) ? 0.0 : f.delv_eta[el + sy], lm, ms);
-        const double phizeta = mono_phi(dvzeta, (k == 0) ? dvzeta : f.delv_zeta[el - sz],
-                                        (k == nx - 1) ? 0.0 : f.delv_zeta[el + sz], lm, ms);
-        const double vdov = f.vdov[el];
-        const double vnew = f.vnew[el];
-        const double volo = f.volo[el];
-        double qlin, qquad;
-        if (vdov > 0.) {
-            qlin = 0.;
-            qquad = 0.;
-        } else {
-            double dxi = dvxi * f.delx_xi[el];
-            double deta = dveta * f.delx_eta[el];
-            double dzeta = dvzeta * f.delx_zeta[el];
-            if (dxi > 0.) dxi = 0.;
-            if (deta > 0.) deta = 0.;
-            if (dzeta > 0.) dzeta = 0.;
-            const double rho = f.elem_mass[el] / (volo * vnew);
-            qlin = -prm.qlc_monoq * rho *
-                   (dxi * (1. - phixi) + deta * (1. - phieta) + dzeta * (1. - phizeta));
-            qquad = prm.qqc_monoq * rho *
-                    (dxi * dxi * (1. - phixi * phixi) + deta * deta * (1. - phieta * phieta) +
-                     dzeta * dzeta * (1. - phizeta * phizeta));
-        }
+__device__ __forceinline__ void q_inputs(const DevFields &f, const PhysParams &prm,
+                                         int el, int i, int j, int k, QInputs &in)
+{
+    const int nx = f.nx;
+    const int sy = nx, sz = nx * nx;
+    const double dvxi = f.delv_xi[el], dveta = f.delv_eta[el], dvzeta = f.delv_zeta[el];
+    const double lm = prm.monoq_limiter_mult, ms = prm.monoq_max_slope;
+    in.phi[0] = mono_phi(dvxi, (i == 0) ? dvxi : f.delv_xi[el - 1],
+                         (i == nx - 1) ? 0.0 : f.delv_xi[el + 1], lm, ms);
+    in.phi[1] = mono_phi(dveta, (j == 0) ? dveta : f.delv_eta[el - sy],
+                         (j == nx - 1) ? 0.0 : f.delv_eta[el + sy], lm, ms);
+    in.phi[2] = mono_phi(dvzeta, (k == 0) ? dvzeta : f.delv_zeta[el - sz],
+                         (k == nx - 1) ? 0.0 : f.delv_zeta[el + sz], lm, ms);
+    in.vdov = f.vdov[el];
+    in.vnew = f.vnew[el];
+    in.volo = f.volo[el];
+    in.du[0] = in.du[1] = in.du[2] = 0.0;
+    in.rho = 0.0;
+    if (!(in.vdov > 0.)) {
+        double dxi = dvxi * f.delx_xi[el];
+        double deta = dveta * f.delx_eta[el];
+        double dzeta = dvzeta * f.delx_zeta[el];
+        if (dxi > 0.) dxi = 0.;
+        if (deta > 0.) deta = 0.;
+        if (dzeta > 0.) dzeta = 0.;
+        in.du[0] = dxi;
+        in.du[1] = deta;
+        in.du[2] = dzeta;
+        in.rho = f.elem_mass[el] / (in.volo * in.vnew);
+    }
+}
+
+// linear and quadratic Q of the monotonic limiter (:2289-2316)
+__device__ __forceinline__ void monotonic_q(const PhysParams &prm, const QInputs &in,
+                                            double &qlin, double &qquad)
+{
+    if (in.vdov > 0.) {
+        qlin = 0.;
+        qquad = 0.;
+    } else {
+        const double dxi = in.du[0], deta = in.du[1], dzeta = in.du[2];
+        const double phixi = in.phi[0], phieta = in.phi[1], phizeta = in.phi[2];
+        qlin = -prm.qlc_monoq * in.rho *
+               (dxi * (1. - phixi) + deta * (1. - phieta) + dzeta * (1. - phizeta));
+        qquad = prm.qqc_monoq * in.rho *
+                (dxi * dxi * (1. - phixi * phixi) + deta * deta * (1. - phieta * phieta) +
+                 dzeta * dzeta * (1. - phizeta * phizeta));
+    }
+}
+
+// Everything after ql / qq for one element: qstop, the volume clamps, the
+// EOS with the region repetitions, the volume update and the element's
+// Courant / hydro candidates.
+__device__ __forceinline__ void eos_dt_elem(const DevFields &f, DevScalars *s,
+                                            const PhysParams &prm, int el, int rep,
+                                            double vdov, double vnew, double qlin,
+                                            double qquad, unsigned long long &cand_c,
+                                            unsigned long long &cand_h)
+{
+    {
         f.ql[el] = qlin;
         f.qq[el] = qquad;
 
@@ -454,14 +480,18 @@ __device__ __forceinline__ void q_eos_dt_block(const DevFields &f, DevScalars *s
         EosOut o;
         o.p = o.e = o.q = o.ss = 0.0;
         // the region's repetitions redo the same evaluation, as the
-        // reference does (lulesh-2.0.3/lulesh.cc:2267); the empty asm hides
-        // that the inputs are unchanged so every repetition is computed
+        // reference does (lulesh-2.0.3/lulesh.cc:2267).  The empty asm
+        // statements hide that the inputs are unchanged and mark every
+        // repetition's result as used, so each one is computed; two
+        // repetitions at a time give the scheduler independent work.
+#pragma unroll 2
         for (int r = 0; r < rep; ++r) {
             double a0 = vnewc, a1 = e_old, a2 = delvc, a3 = p_old, a4 = q_old;
             double a5 = qquad, a6 = qlin;
             asm volatile("" : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(a4),
                          "+v"(a5), "+v"(a6));
             o = eos_element(a0, a1, a2, a3, a4, a5, a6, ec);
+            asm volatile("" ::"v"(o.p), "v"(o.e), "v"(o.q), "v"(o.ss));
         }
         f.p[el] = o.p;
         f.e[el] = o.e;
@@ -492,15 +522,44 @@ __device__ __forceinline__ void q_eos_dt_block(const DevFields &f, DevScalars *s
             cand_h = as_bits(dtdvov);
         }
     }
-    // fold the block minima into the cycle's accumulators; they are applied
-    // to dtcourant / dthydro after the kernel (apply_constraints).  No fence
-    // is needed: the next kernel on the stream sees the atomics' results.
+}
+
+// fold the block minima into the cycle's accumulators; they are applied to
+// dtcourant / dthydro after the kernel (apply_constraints).  No fence is
+// needed: the next kernel on the stream sees the atomics' results.
+__device__ __forceinline__ void fold_minima(DevScalars *s, unsigned long long cand_c,
+                                            unsigned long long cand_h,
+                                            unsigned long long *lds_c,
+                                            unsigned long long *lds_h)
+{
     const unsigned long long mc = block_min(cand_c, lds_c);
     const unsigned long long mh = block_min(cand_h, lds_h);
     if (threadIdx.x == 0) {
         if (mc < as_bits(kBig)) atomicMin(&s->acc_courant, mc);
         if (mh < as_bits(kBig)) atomicMin(&s->acc_hydro, mh);
     }
+}
+
+// one block of 256 elements in schedule order; every thread of the block
+// calls it (the minima are reduced over the block)
+__device__ __forceinline__ void q_eos_dt_block(const DevFields &f, DevScalars *s,
+                                               const PhysParams &prm, int slot,
+                                               unsigned long long *lds_c,
+                                               unsigned long long *lds_h)
+{
+    unsigned long long cand_c = as_bits(kBig), cand_h = as_bits(kBig);
+    if (slot < f.num_elem) {
+        const int el = f.eos_order[slot];
+        const int rep = f.eos_rep[slot];
+        int i, j, k;
+        elem_ijk(f, el, i, j, k);
+        QInputs in;
+        q_inputs(f, prm, el, i, j, k, in);
+        double qlin, qquad;
+        monotonic_q(prm, in, qlin, qquad);
+        eos_dt_elem(f, s, prm, el, rep, in.vdov, in.vnew, qlin, qquad, cand_c, cand_h);
+    }
+    fold_minima(s, cand_c, cand_h, lds_c, lds_h);
 }
 
 }  // namespace
@@ -548,6 +607,150 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_batch(const BatchMember *membe
         }
         q_eos_dt_block(m.f, m.s, prm, w.y + int(threadIdx.x), lds_c, lds_h);
     }
+}
+
+// ---------------------------------------------------------------------------
+// The learned closure.  The network sees twelve dimensionless inputs of an
+// element, formed from what the monotonic limiter works from:
+//   0-2   the limiter values phi of the xi, eta, zeta directions
+//   3-5   the velocity jumps du (delv * delx, <= 0) over the sound speed c,
+//         as du / (c + |du|)
+//   6-8   delx of each direction over their mean hbar
+//   9     vdov * hbar / (c + |vdov| hbar)
+//   10    log of the relative volume vnew (the density is m / (V0 vnew))
+//   11    log of the sound speed c of the last EOS evaluation
+// and returns two factors on the monotonic limiter's linear and quadratic
+// Q of the element, ql = m_l ql_limiter, qq = m_q qq_limiter, each clamped
+// to [0, 3] (m = 1 is the plain limiter).  The element keeps the plain
+// limiter when an input lies more than z_max standard deviations from the
+// training mean, when its volume or energy is not positive, or when it has
+// no compression to limit.
+// ---------------------------------------------------------------------------
+namespace {
+
+__device__ __forceinline__ void learned_inputs(const DevFields &f, const QInputs &in,
+                                               int el, float feat[kMlpIn],
+                                               double &ql_unlim, double &qq_unlim,
+                                               bool &usable)
+{
+    const double c = f.ss[el];
+    const double hbar = (f.delx_xi[el] + f.delx_eta[el] + f.delx_zeta[el]) * (1.0 / 3.0);
+    const double delx[3] = {f.delx_xi[el], f.delx_eta[el], f.delx_zeta[el]};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        feat[d] = float(in.phi[d]);
+        feat[3 + d] = float(in.du[d] / (c + fabs(in.du[d]) + 1e-300));
+        feat[6 + d] = float(delx[d] / hbar);
+    }
+    feat[9] = float(in.vdov * hbar / (c + fabs(in.vdov) * hbar + 1e-300));
+    feat[10] = float(log(in.vnew > 0.0 ? in.vnew : 1.0));
+    feat[11] = float(log(c > 0.0 ? c : 1e-300));
+    const double sdu = in.du[0] + in.du[1] + in.du[2];
+    const double sdu2 = in.du[0] * in.du[0] + in.du[1] * in.du[1] + in.du[2] * in.du[2];
+    ql_unlim = -in.rho * sdu;
+    qq_unlim = in.rho * sdu2;
+    usable = !(in.vdov > 0.) && sdu < 0.0 && in.vnew > 0.0 && f.e[el] >= 0.0 &&
+             hbar > 0.0 && c > 0.0;
+}
+
+}  // namespace
+
+// features of every element for training: feat[el][12] and
+// q[el][4] = (unlimited ql / qlc, unlimited qq / qqc, plain ql, plain qq);
+// the unlimited values are 0 where the element has no compression
+__global__ __launch_bounds__(256) void k_q_features(DevFields f, PhysParams prm,
+                                                    float *feat, double *q)
+{
+    const int el = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
+    if (el >= f.num_elem) {
+        return;
+    }
+    int i, j, k;
+    elem_ijk(f, el, i, j, k);
+    QInputs in;
+    q_inputs(f, prm, el, i, j, k, in);
+    double qlin, qquad;
+    monotonic_q(prm, in, qlin, qquad);
+    float x[kMlpIn];
+    double ql_u, qq_u;
+    bool usable;
+    learned_inputs(f, in, el, x, ql_u, qq_u, usable);
+    for (int t = 0; t < kMlpIn; ++t) {
+        feat[size_t(el) * kMlpIn + t] = x[t];
+    }
+    q[size_t(el) * 4 + 0] = usable ? ql_u : 0.0;
+    q[size_t(el) * 4 + 1] = usable ? qq_u : 0.0;
+    q[size_t(el) * 4 + 2] = qlin;
+    q[size_t(el) * 4 + 3] = qquad;
+}
+
+// k_q_eos_dt with the learned closure in place of the limiter: every wave
+// evaluates the network on its 64 elements on the matrix cores
+__global__ __launch_bounds__(256) void k_q_eos_dt_learned(DevFields f, DevScalars *s,
+                                                          PhysParams prm,
+                                                          const MlpFrags *net,
+                                                          float z_max)
+{
+    __shared__ unsigned long long lds_c[4], lds_h[4];
+    __shared__ int skip;
+    __shared__ MlpWaveLds lds[4];
+    if (threadIdx.x == 0) {
+        skip = (s->active == 0 || s->error != 0) ? 1 : 0;
+    }
+    __syncthreads();
+    if (skip) {
+        return;
+    }
+    const int lane = int(threadIdx.x) & 63;
+    MlpWaveLds &L = lds[threadIdx.x >> 6];
+    const MlpFrags &F = *net;
+    const int chunk = int(gridDim.x) - 1 - int(blockIdx.x);
+    const int slot = chunk * int(blockDim.x) + int(threadIdx.x);
+    const bool valid = slot < f.num_elem;
+    int el = 0, rep = 0;
+    QInputs in;
+    double qlin = 0.0, qquad = 0.0, ql_u = 0.0, qq_u = 0.0;
+    bool use_net = false;
+    float x[kMlpIn];
+#pragma unroll
+    for (int t = 0; t < kMlpIn; ++t) {
+        x[t] = 0.f;
+    }
+    if (valid) {
+        el = f.eos_order[slot];
+        rep = f.eos_rep[slot];
+        int i, j, k;
+        elem_ijk(f, el, i, j, k);
+        q_inputs(f, prm, el, i, j, k, in);
+        monotonic_q(prm, in, qlin, qquad);
+        learned_inputs(f, in, el, x, ql_u, qq_u, use_net);
+    }
+    // standardise; out-of-range inputs keep the plain limiter
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        float z = 0.f;
+        if (t < kMlpIn) {
+            z = (x[t] - F.in_mean[t]) * F.in_inv_std[t];
+            if (!(fabsf(z) <= z_max)) {
+                use_net = false;
+                z = 0.f;
+            }
+        }
+        L.x[lane][t] = to_bf16(z);
+    }
+    __builtin_amdgcn_wave_barrier();
+    mlp_forward_wave(F, L, lane);
+    if (use_net) {
+        const float ml = fminf(fmaxf(L.y[lane][0], 0.f), 3.f);
+        const float mq = fminf(fmaxf(L.y[lane][1], 0.f), 3.f);
+        qlin = double(ml) * qlin;
+        qquad = double(mq) * qquad;
+    }
+    unsigned long long cand_c = as_bits(kBig), cand_h = as_bits(kBig);
+    if (valid) {
+        eos_dt_elem(f, s, prm, el, rep, in.vdov, in.vnew, qlin, qquad, cand_c, cand_h);
+    }
+    fold_minima(s, cand_c, cand_h, lds_c, lds_h);
 }
 
 // applies the minima of the last cycle: 2.0 starts every cycle from 1e20

@@ -165,6 +165,12 @@ hipError_t launch_force_sweep_batch(const SweepShape &shape, int blocks,
                                     const PhysParams &prm, hipStream_t stream);
 bool force_sweep_shape_supported(const SweepShape &shape);
 
+// The learned closure (network in lulesh_mlp.hpp).
+struct MlpFrags;
+__global__ void k_q_features(DevFields f, PhysParams prm, float *feat, double *q);
+__global__ void k_q_eos_dt_learned(DevFields f, DevScalars *s, PhysParams prm,
+                                   const MlpFrags *net, float z_max);
+
 // The mixed-precision phases (the time step kernels are shared).
 __global__ void k_kinematics_q_mixed(DevFieldsMixed f, DevScalars *s, PhysParams prm);
 __global__ void k_q_eos_dt_mixed(DevFieldsMixed f, DevScalars *s, PhysParams prm);

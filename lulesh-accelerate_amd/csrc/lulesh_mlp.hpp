@@ -192,3 +192,7 @@ __device__ __forceinline__ void mlp_forward_wave(const MlpFrags &F, MlpWaveLds &
 }
 
 }  // namespace lulesh
+
+// the packed fragments of a network (lulesh_mlp.hip)
+struct lulesh_mlp;
+const lulesh::MlpFrags *lulesh_mlp_device_frags(const lulesh_mlp *n);

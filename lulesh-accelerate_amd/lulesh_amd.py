@@ -61,7 +61,7 @@ _lib = None
 # every symbol declared in include/lulesh_mi355x.h
 EXPORTS = (
     "lulesh_config_default", "lulesh_create", "lulesh_destroy",
-    "lulesh_create_mixed", "lulesh_is_mixed", "lulesh_step",
+    "lulesh_create_mixed", "lulesh_create_scaled", "lulesh_is_mixed", "lulesh_step",
     "lulesh_run", "lulesh_upload_state", "lulesh_download_state",
     "lulesh_get_field", "lulesh_set_field", "lulesh_get_stats",
     "lulesh_set_scalars", "lulesh_set_parameter", "lulesh_set_sweep_shape",
@@ -77,7 +77,8 @@ EXPORTS = (
     "lulesh_batch_last_error",
     "lulesh_mlp_param_count", "lulesh_mlp_create", "lulesh_mlp_destroy",
     "lulesh_mlp_get_params", "lulesh_mlp_eval", "lulesh_mlp_gradient",
-    "lulesh_mlp_train", "lulesh_mlp_last_error",
+    "lulesh_mlp_train", "lulesh_mlp_last_error", "lulesh_set_learned_q",
+    "lulesh_q_features",
 )
 
 
@@ -102,6 +103,8 @@ def lib() -> ctypes.CDLL:
     L.lulesh_destroy.restype = None
     L.lulesh_create_mixed.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(P)]
     L.lulesh_create_mixed.restype = I
+    L.lulesh_create_scaled.argtypes = [ctypes.POINTER(Config), D, I, ctypes.POINTER(P)]
+    L.lulesh_create_scaled.restype = I
     L.lulesh_is_mixed.argtypes = [P]
     L.lulesh_is_mixed.restype = I
     L.lulesh_step.argtypes = [P, I, IP]
@@ -189,6 +192,10 @@ def lib() -> ctypes.CDLL:
     L.lulesh_mlp_train.restype = I
     L.lulesh_mlp_last_error.argtypes = [P]
     L.lulesh_mlp_last_error.restype = ctypes.c_char_p
+    L.lulesh_set_learned_q.argtypes = [P, P, D]
+    L.lulesh_set_learned_q.restype = I
+    L.lulesh_q_features.argtypes = [P, FP, DP]
+    L.lulesh_q_features.restype = I
     _lib = L
     return L
 
@@ -208,16 +215,23 @@ class Domain:
 
     def __init__(self, nx: int = 30, ic_mode: int = 2, num_reg: int = 11,
                  balance: int = 1, cost: int = 1, max_cycles: int = 9999999,
-                 stoptime: float = 1.0e-2, mixed: bool = False):
-        """mixed=True builds the mixed-precision form (lulesh_create_mixed)."""
+                 stoptime: float = 1.0e-2, mixed: bool = False,
+                 efactor: float = 1.0):
+        """mixed=True builds the mixed-precision form (lulesh_create_mixed);
+        efactor multiplies the initial origin energy."""
         L = lib()
         cfg = default_config()
         cfg.nx, cfg.ic_mode, cfg.num_reg = nx, ic_mode, num_reg
         cfg.balance, cfg.cost = balance, cost
         cfg.max_cycles, cfg.stoptime = max_cycles, stoptime
         h = ctypes.c_void_p()
-        create = L.lulesh_create_mixed if mixed else L.lulesh_create
-        rc = create(ctypes.byref(cfg), ctypes.byref(h))
+        if efactor != 1.0:
+            rc = L.lulesh_create_scaled(ctypes.byref(cfg), float(efactor), int(mixed),
+                                        ctypes.byref(h))
+        elif mixed:
+            rc = L.lulesh_create_mixed(ctypes.byref(cfg), ctypes.byref(h))
+        else:
+            rc = L.lulesh_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != OK:
             raise LuleshError(rc, "lulesh_create failed")
         self.h = h
@@ -340,6 +354,23 @@ class Domain:
         self._check(lib().lulesh_upload_state(
             self.h, *[_dp(b) for b in bufs], float(st["time"]),
             float(st["deltatime"])))
+
+    # -- learned closure ------------------------------------------------
+    def set_learned_q(self, net, z_max: float = 4.0) -> None:
+        """Uses the network `net` (an Mlp, kept alive by this domain) in
+        place of the monotonic limiter; None restores the limiter."""
+        self._net = net
+        self._check(lib().lulesh_set_learned_q(self.h, net.h if net else None,
+                                               float(z_max)))
+
+    def q_features(self) -> tuple[np.ndarray, np.ndarray]:
+        """(features[num_elem, 12], q[num_elem, 4]) of the current state."""
+        f = np.empty((self.num_elem, 12), np.float32)
+        q = np.empty((self.num_elem, 4), np.float64)
+        rc = lib().lulesh_q_features(self.h, _fp(f), _dp(q))
+        if rc not in (OK, VOLUME_ERROR):
+            self._check(rc)
+        return f, q
 
     # -- integral diagnostics -------------------------------------------
     def total_energy(self) -> float:
