@@ -7,11 +7,13 @@ balance 1, cost 1, seeded region assignment) and LULESH 2.0 initial
 conditions, in float64.  The domain is built once and stays resident on the
 GPU; the timed region is K consecutive cycles after W warm-up cycles.
 
-Prints one JSON line: zones*cycles/s of the timed cycles, the roofline of
-the dominant kernel (algorithmic bytes per launch over its mean duration,
-from HIP events on the domain's stream) and the reference LULESH 2.0.3
-(oracle/_ref/lulesh2, built from the reference sources) timed on this
-host's cores on a 20-cycle sample of the same workload.
+Prints one JSON line: zones*cycles/s of the timed cycles (queued back to
+back, no events between kernels), the roofline of the dominant kernel
+(algorithmic bytes per launch over its mean duration, from HIP events
+around every launch on the domain's stream during 50 further cycles) and
+the reference LULESH 2.0.3 (oracle/_ref/lulesh2, built from the reference
+sources) timed on this host's cores on a 20-cycle sample of the same
+workload.
 
 The "forms" object holds the other forms of the step on their own
 workloads: the ensemble (64 Sedov members of uneven size, each to t=1e-2)
@@ -248,20 +250,31 @@ def main() -> int:
     import lulesh_amd
 
     nx, K, W = args.nx, args.steps, args.warmup
+    KT = min(K, 50)  # cycles of the per-kernel breakdown, after the timed ones
     d = lulesh_amd.Domain(nx, ic_mode=2, num_reg=11, balance=1, cost=1,
-                          max_cycles=W + K)
+                          max_cycles=W + K + KT)
     if W:
         d.step(W)
-    d.set_kernel_timing(True)
+    # the timed region: K cycles queued back to back, no events between the
+    # kernels; lulesh_step is synchronous, so the device is idle before and
+    # after
     t0 = time.perf_counter()
-    rc, done = d.step(K)  # synchronous: device idle before and after
+    rc, done = d.step(K)
     wall = time.perf_counter() - t0
     if rc != 0 or done != K:
         print(f"bench: step returned rc={rc}, {done} of {K} cycles",
               file=sys.stderr)
         return 1
-    times = d.kernel_times()
     st = d.stats()
+    # per-kernel durations: HIP events around every launch on the domain's
+    # stream, over KT further cycles
+    d.set_kernel_timing(True)
+    rc, done = d.step(KT)
+    if rc != 0 or done != KT:
+        print(f"bench: breakdown step returned rc={rc}, {done} of {KT} cycles",
+              file=sys.stderr)
+        return 1
+    times = d.kernel_times()
     d.close()
 
     ne, nn = nx ** 3, (nx + 1) ** 3
@@ -303,7 +316,8 @@ def main() -> int:
                            "roofline_zones_cycles_per_s": roof_value,
                            "frac": value / roof_value},
         "kernels_mean_ms": {k: round(v, 6) for k, v in per_kernel.items()},
-        "gpu_event_ms_per_step": sum(ms for ms, _ in times.values()) / K,
+        "gpu_event_ms_per_step": sum(ms for ms, _ in times.values()) / KT,
+        "breakdown_cycles": KT,
     }
     if not args.no_forms:
         result["forms"] = {"batched": bench_batched(lulesh_amd, value),

@@ -165,6 +165,20 @@ int lulesh_get_regions(lulesh_domain *d, int *reg_num_list);
 int lulesh_symmetry(lulesh_domain *d, double *max_abs_diff,
                     double *total_abs_diff, double *max_rel_diff);
 
+/* The state as a file for visual verification: replaces writeDomain
+ * (cbits/visit.cc:147-175, bound by src/VisIt.hs:71-86), with the same
+ * arguments, writing a legacy VTK unstructured grid (VisIt and ParaView read
+ * it; the Silo library is not available): hexahedra of the num_elem = n^3
+ * cube, node velocities, cell e, p, v, q, time and cycle.  The arrays are
+ * host arrays in the reference's order. */
+int lulesh_write_domain_arrays(const char *path, int num_elem, int step, double time,
+                               const double *x, const double *y, const double *z,
+                               const double *xd, const double *yd, const double *zd,
+                               const double *e, const double *p, const double *v,
+                               const double *q);
+/* The same for a domain's current state (downloaded first). */
+int lulesh_write_domain(lulesh_domain *d, const char *path);
+
 /* Per-kernel timing with HIP events on the domain's stream.  While enabled,
  * every launch of lulesh_step is bracketed by events; lulesh_kernel_times
  * returns, per kernel, the accumulated milliseconds and launch count. */

@@ -23,7 +23,7 @@ HARNESS = os.path.join(PKG, "lulesh_mi355x")
 
 HIP_SOURCES = ["lulesh_kernels.hip", "lulesh_force.hip", "lulesh_mixed.hip",
                "lulesh_abi.hip", "lulesh_batch.hip", "lulesh_mlp.hip",
-               "lulesh_setup.cpp",
+               "lulesh_setup.cpp", "lulesh_output.cpp",
                "lulesh_mesh.cpp"]
 HEADERS = ["lulesh_kernels.hpp", "lulesh_elem.hpp", "lulesh_cbrt.h",
            "lulesh_mesh.hpp", "lulesh_setup.hpp", "lulesh_mlp.hpp"]

@@ -355,7 +355,7 @@ __global__ __launch_bounds__(TX *TY, W) void k_force_sweep(DevFields f, DevScala
 // leave their tiles at once (sweep_tile reads their flags).
 template <int TX, int TY, int W>
 __global__ __launch_bounds__(TX *TY, W) void k_force_sweep_batch(
-    const BatchMember *members, const BatchWork *work, PhysParams prm, int nz)
+    const BatchMember *__restrict__ members, const BatchWork *__restrict__ work, PhysParams prm, int nz)
 {
     const int n = work->num_tiles;
     for (int t = int(blockIdx.x); t < n; t += int(gridDim.x)) {

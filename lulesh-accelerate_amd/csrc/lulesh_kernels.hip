@@ -145,8 +145,8 @@ __global__ void k_time_increment(DevScalars *s, int ic_mode)
 }
 
 // one thread per member
-__global__ void k_time_increment_batch(const BatchMember *members,
-                                       const BatchWork *work, int ic_mode)
+__global__ void k_time_increment_batch(const BatchMember *__restrict__ members,
+                                       const BatchWork *__restrict__ work, int ic_mode)
 {
     const int n = work->num_members;
     for (int m = int(threadIdx.x); m < n; m += int(blockDim.x)) {
@@ -287,8 +287,8 @@ __global__ __launch_bounds__(256) void k_kinematics_q(DevFields f, DevScalars *s
 }
 
 // persistent blocks over the element chunks of all members
-__global__ __launch_bounds__(256) void k_kinematics_q_batch(const BatchMember *members,
-                                                            const BatchWork *work,
+__global__ __launch_bounds__(256) void k_kinematics_q_batch(const BatchMember *__restrict__ members,
+                                                            const BatchWork *__restrict__ work,
                                                             PhysParams prm)
 {
     const int n = work->num_chunks;
@@ -587,8 +587,8 @@ __global__ __launch_bounds__(256) void k_q_eos_dt(DevFields f, DevScalars *s,
 
 // persistent blocks over the element chunks of all members; a chunk lies
 // inside one member, so its minima go to that member's accumulators
-__global__ __launch_bounds__(256) void k_q_eos_dt_batch(const BatchMember *members,
-                                                        const BatchWork *work,
+__global__ __launch_bounds__(256) void k_q_eos_dt_batch(const BatchMember *__restrict__ members,
+                                                        const BatchWork *__restrict__ work,
                                                         PhysParams prm)
 {
     __shared__ unsigned long long lds_c[4], lds_h[4];

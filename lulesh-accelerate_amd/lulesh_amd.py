@@ -78,7 +78,7 @@ EXPORTS = (
     "lulesh_mlp_param_count", "lulesh_mlp_create", "lulesh_mlp_destroy",
     "lulesh_mlp_get_params", "lulesh_mlp_eval", "lulesh_mlp_gradient",
     "lulesh_mlp_train", "lulesh_mlp_last_error", "lulesh_set_learned_q",
-    "lulesh_q_features",
+    "lulesh_q_features", "lulesh_write_domain_arrays", "lulesh_write_domain",
 )
 
 
@@ -196,6 +196,10 @@ def lib() -> ctypes.CDLL:
     L.lulesh_set_learned_q.restype = I
     L.lulesh_q_features.argtypes = [P, FP, DP]
     L.lulesh_q_features.restype = I
+    L.lulesh_write_domain_arrays.argtypes = [ctypes.c_char_p, I, I, D] + [DP] * 10
+    L.lulesh_write_domain_arrays.restype = I
+    L.lulesh_write_domain.argtypes = [P, ctypes.c_char_p]
+    L.lulesh_write_domain.restype = I
     _lib = L
     return L
 
@@ -371,6 +375,10 @@ class Domain:
         if rc not in (OK, VOLUME_ERROR):
             self._check(rc)
         return f, q
+
+    def write(self, path: str) -> None:
+        """The state as a VTK file (lulesh_write_domain)."""
+        self._check(lib().lulesh_write_domain(self.h, path.encode()))
 
     # -- integral diagnostics -------------------------------------------
     def total_energy(self) -> float:
@@ -636,3 +644,13 @@ class Mlp:
                                            int(epochs), int(batch), float(lr), int(seed),
                                            ctypes.byref(loss)))
         return loss.value
+
+
+def write_domain_arrays(path: str, step: int, time: float, st: dict) -> None:
+    """writeDomain on host arrays (x, y, z, xd, yd, zd, e, p, v, q)."""
+    arrs = [np.ascontiguousarray(st[k], np.float64)
+            for k in ("x", "y", "z", "xd", "yd", "zd", "e", "p", "v", "q")]
+    rc = lib().lulesh_write_domain_arrays(path.encode(), len(arrs[6]), int(step),
+                                          float(time), *[_dp(a) for a in arrs])
+    if rc != OK:
+        raise LuleshError(rc, "lulesh_write_domain_arrays failed")

@@ -43,13 +43,18 @@ def main() -> int:
     ap.add_argument("--cycles", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--copy-bytes", type=int, default=0)
+    ap.add_argument("--mixed", action="store_true",
+                    help="also run the mixed-precision step on the same workload")
     args = ap.parse_args()
     if args.cycles > 0:
-        d = lulesh_amd.Domain(args.nx, ic_mode=2, max_cycles=args.warmup + args.cycles)
-        d.step(args.warmup)
-        rc, n = d.step(args.cycles)
-        print(f"lulesh {args.nx}^3: {n} cycles, rc {rc}", flush=True)
-        d.close()
+        for mixed in ((False, True) if args.mixed else (False,)):
+            d = lulesh_amd.Domain(args.nx, ic_mode=2, max_cycles=args.warmup + args.cycles,
+                                  mixed=mixed)
+            d.step(args.warmup)
+            rc, n = d.step(args.cycles)
+            print(f"lulesh {args.nx}^3{' mixed' if mixed else ''}: {n} cycles, rc {rc}",
+                  flush=True)
+            d.close()
     if args.copy_bytes > 0:
         device_copy(args.copy_bytes)
         print(f"copied {args.copy_bytes} bytes device to device", flush=True)
