@@ -105,6 +105,35 @@ int lulesh_create_mixed(const lulesh_config *cfg, lulesh_domain **out);
  * dt derived from it): one member of a seeded ensemble of Sedov problems. */
 int lulesh_create_scaled(const lulesh_config *cfg, double efactor, int mixed,
                          lulesh_domain **out);
+/* An unstructured hexahedral mesh (the .lmesh input of
+ * reference/CUDA/lulesh-kepler-singlegpu/lulesh.cu:728-925) in LULESH 1.0
+ * semantics (cfg->ic_mode must be LULESH_IC_1_0; only stoptime and
+ * max_cycles are read; NULL: the defaults with 1.0 ICs -- pass a config).
+ * The element face masks are derived from the node sets as the reference's
+ * reader derives them: a face whose four nodes lie on the symmetry sets is
+ * a symmetry face, and so is one whose four nodes lie on the free-surface
+ * set; the initial energy goes into the first element whose mask is the
+ * three low symmetry faces.  Corner forces are gathered per node in
+ * ascending element order (reference/C/LULESH-OMP.cc:209-268).  The plane-0
+ * symmetry check, sweep shapes and the learned closure do not apply.
+ * lulesh_create_lmesh reads the file: node and element counts, the node
+ * coordinates, 8 nodes per element, 6 face neighbours per element
+ * (xi-, xi+, eta-, eta+, zeta-, zeta+), then the x, y, z symmetry node sets
+ * and the free-surface node set, each preceded by its size.
+ * lulesh_create_mesh takes the same lists as arrays. */
+int lulesh_create_lmesh(const char *path, const lulesh_config *cfg, lulesh_domain **out);
+int lulesh_create_mesh(int num_elem, int num_node, const double *x, const double *y,
+                       const double *z, const int *nodelist, const int *neighbors,
+                       int nsx, const int *symm_x, int nsy, const int *symm_y, int nsz,
+                       const int *symm_z, int nfree, const int *free_surf,
+                       const lulesh_config *cfg, lulesh_domain **out);
+/* The element holding the initial energy (0 on the cube). */
+int lulesh_origin_element(const lulesh_domain *d);
+/* The element corners [num_elem][8] (node indices, bottom face
+ * counter-clockwise, then the top face): the nodelist of
+ * reference/C/LULESH-OMP.cc:2996-3012 for the cube, the file's lists for a
+ * mesh read by lulesh_create_lmesh (lulesh.cu:728-848). */
+int lulesh_get_nodelist(lulesh_domain *d, int *nodelist);
 /* 1 for a domain made by lulesh_create_mixed, else 0. */
 int lulesh_is_mixed(const lulesh_domain *d);
 
@@ -176,7 +205,8 @@ int lulesh_write_domain_arrays(const char *path, int num_elem, int step, double 
                                const double *xd, const double *yd, const double *zd,
                                const double *e, const double *p, const double *v,
                                const double *q);
-/* The same for a domain's current state (downloaded first). */
+/* The same for a domain's current state (downloaded first), the cube or an
+ * unstructured mesh (its own connectivity). */
 int lulesh_write_domain(lulesh_domain *d, const char *path);
 
 /* Per-kernel timing with HIP events on the domain's stream.  While enabled,

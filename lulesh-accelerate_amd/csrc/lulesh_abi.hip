@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -29,12 +30,14 @@ enum KernelId {
     KID_KINEMATICS_Q,
     KID_Q_EOS_DT,
     KID_DT_FINISH,
+    KID_CORNER_FORCES_U,  // unstructured meshes: element corner forces
+    KID_NODES_U,          //   and the node gather and update
     KID_COUNT
 };
 
 const char *const kKernelNames[KID_COUNT] = {
     "k_time_increment", "k_force_sweep", "k_kinematics_q", "k_q_eos_dt",
-    "k_dt_finish",
+    "k_dt_finish", "k_corner_forces_u", "k_nodes_u",
 };
 
 // default launch shape of the force sweep (32 x 8 element columns, 16 node
@@ -63,6 +66,10 @@ struct lulesh_domain {
     std::vector<int32_t> reg_num_list;
 
     bool mixed = false;   // float32 element-local arithmetic (fm), else f
+    bool unstructured = false;  // an explicit-connectivity mesh (mesh), f
+    DevMeshU mesh{};
+    int origin = 0;             // the element holding the initial energy
+    std::vector<void *> bufs;   // device allocations of an unstructured domain
     const MlpFrags *learned = nullptr;  // learned-Q network, if set
     float learned_z_max = 0.f;
     DevFields f{};
@@ -138,8 +145,44 @@ int launch(lulesh_domain *d, int kid, F &&body)
     return LULESH_OK;
 }
 
+int queue_cycle_u(lulesh_domain *d)
+{
+    const unsigned ge = unsigned((d->num_elem + kBlock - 1) / kBlock);
+    const unsigned gn = unsigned((d->num_node + kBlock - 1) / kBlock);
+    hipStream_t st = d->stream;
+    DevScalars *s = d->s;
+    const DevFields &f = d->f;
+    const DevMeshU &m = d->mesh;
+    const PhysParams &p = d->prm;
+    int rc;
+    if ((rc = launch(d, KID_TIME_INCREMENT, [&] {
+             hipLaunchKernelGGL(k_time_increment, dim3(1), dim3(64), 0, st, s, p.ic_mode);
+         })))
+        return rc;
+    if ((rc = launch(d, KID_CORNER_FORCES_U, [&] {
+             hipLaunchKernelGGL(k_corner_forces_u, dim3(ge), dim3(kBlock), 0, st, f, m, s, p);
+         })))
+        return rc;
+    if ((rc = launch(d, KID_NODES_U, [&] {
+             hipLaunchKernelGGL(k_nodes_u, dim3(gn), dim3(kBlock), 0, st, f, m, s, p);
+         })))
+        return rc;
+    if ((rc = launch(d, KID_KINEMATICS_Q, [&] {
+             hipLaunchKernelGGL(k_kinematics_q_u, dim3(ge), dim3(kBlock), 0, st, f, m, s, p);
+         })))
+        return rc;
+    if ((rc = launch(d, KID_Q_EOS_DT, [&] {
+             hipLaunchKernelGGL(k_q_eos_dt_u, dim3(ge), dim3(kBlock), 0, st, f, m, s, p);
+         })))
+        return rc;
+    return LULESH_OK;
+}
+
 int queue_cycle(lulesh_domain *d)
 {
+    if (d->unstructured) {
+        return queue_cycle_u(d);
+    }
     const unsigned ge = unsigned((d->num_elem + kBlock - 1) / kBlock);
     hipStream_t st = d->stream;
     DevScalars *s = d->s;
@@ -393,6 +436,223 @@ int lulesh_create_scaled(const lulesh_config *cfg, double efactor, int mixed,
 
 int lulesh_is_mixed(const lulesh_domain *d) { return (d && d->mixed) ? 1 : 0; }
 
+}  // extern "C"
+
+namespace {
+
+// device copy of a host vector, owned by the domain
+template <typename T>
+int upload_vec(lulesh_domain *d, const std::vector<T> &v, T **out, size_t min_count = 0)
+{
+    const size_t n = std::max(v.size(), min_count);
+    void *p = nullptr;
+    HIP_TRY(d, hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    d->bufs.push_back(p);
+    HIP_TRY(d, hipMemsetAsync(p, 0, std::max<size_t>(n, 1) * sizeof(T), d->stream));
+    if (!v.empty()) {
+        HIP_TRY(d, hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice,
+                                  d->stream));
+    }
+    *out = static_cast<T *>(p);
+    return LULESH_OK;
+}
+
+int build_device_domain_u(lulesh_domain *d, const HostMeshU &m)
+{
+    const size_t ne = size_t(m.num_elem), nn = size_t(m.num_node);
+    DevFields &f = d->f;
+    f.nx = 0;
+    f.num_elem = m.num_elem;
+    f.num_node = m.num_node;
+    f.div_nx = make_fastdiv(1u);
+    int rc;
+    const std::vector<double> zn(nn, 0.0), ze(ne, 0.0), ones(ne, 1.0);
+    // node arrays: one copy, updated in place by k_nodes_u
+    if ((rc = upload_vec(d, m.x, &f.x[0])) || (rc = upload_vec(d, m.y, &f.y[0])) ||
+        (rc = upload_vec(d, m.z, &f.z[0])) || (rc = upload_vec(d, zn, &f.xd[0])) ||
+        (rc = upload_vec(d, zn, &f.yd[0])) || (rc = upload_vec(d, zn, &f.zd[0])) ||
+        (rc = upload_vec(d, m.nodal_mass, &f.nodal_mass)))
+        return rc;
+    f.x[1] = f.x[0]; f.y[1] = f.y[0]; f.z[1] = f.z[0];
+    f.xd[1] = f.xd[0]; f.yd[1] = f.yd[0]; f.zd[1] = f.zd[0];
+    double **elem[] = {&f.p, &f.q, &f.ql, &f.qq, &f.ss, &f.vnew, &f.delv,
+                       &f.arealg, &f.vdov, &f.delv_xi, &f.delv_eta, &f.delv_zeta,
+                       &f.delx_xi, &f.delx_eta, &f.delx_zeta};
+    for (double **a : elem) {
+        if ((rc = upload_vec(d, ze, a))) return rc;
+    }
+    if ((rc = upload_vec(d, m.e, &f.e)) || (rc = upload_vec(d, ones, &f.v)) ||
+        (rc = upload_vec(d, m.volo, &f.volo)) || (rc = upload_vec(d, m.volo, &f.elem_mass)))
+        return rc;
+    // one region, one EOS evaluation, element order
+    std::vector<int32_t> order(ne);
+    for (size_t i = 0; i < ne; ++i) order[i] = int32_t(i);
+    const std::vector<uint16_t> rep(ne, 1);
+    int32_t *ord = nullptr;
+    uint16_t *rp = nullptr;
+    if ((rc = upload_vec(d, order, &ord)) || (rc = upload_vec(d, rep, &rp))) return rc;
+    f.eos_order = ord;
+    f.eos_rep = rp;
+    DevMeshU &g = d->mesh;
+    int32_t *a = nullptr;
+    if ((rc = upload_vec(d, m.nodelist, &a))) return rc;
+    g.nodelist = a;
+    if ((rc = upload_vec(d, m.nbr, &a))) return rc;
+    g.nbr = a;
+    if ((rc = upload_vec(d, m.elem_bc, &a))) return rc;
+    g.elem_bc = a;
+    if ((rc = upload_vec(d, m.corner_start, &a))) return rc;
+    g.corner_start = a;
+    if ((rc = upload_vec(d, m.corner_list, &a))) return rc;
+    g.corner_list = a;
+    uint8_t *sy = nullptr;
+    if ((rc = upload_vec(d, m.symm, &sy))) return rc;
+    g.symm = sy;
+    const std::vector<double> corners(8 * ne, 0.0);
+    for (int t = 0; t < 3; ++t) {
+        if ((rc = upload_vec(d, corners, &g.fs[t])) || (rc = upload_vec(d, corners, &g.fh[t])))
+            return rc;
+    }
+    DevScalars *sc = nullptr;
+    const std::vector<DevScalars> one(1);
+    if ((rc = upload_vec(d, one, &sc))) return rc;
+    d->s = sc;
+    HIP_TRY(d, hipStreamSynchronize(d->stream));
+    HostMesh hm;  // the loop scalars of the 1.0 ICs
+    hm.deltatime0 = 1.0e-7;
+    hm.dtfixed = -1.0e-7;
+    return write_scalars(d, initial_scalars(hm, d->cfg.stoptime, d->cfg.max_cycles));
+}
+
+int create_domain_u(const MeshLists &lists, const lulesh_config *cfg, lulesh_domain **out)
+{
+    if (!out) {
+        return LULESH_BAD_ARG;
+    }
+    *out = nullptr;
+    lulesh_config c;
+    lulesh_config_default(&c);
+    if (cfg) {
+        c = *cfg;
+    }
+    if (!(c.stoptime > 0.0) || c.max_cycles < 0 || c.ic_mode != LULESH_IC_1_0) {
+        return LULESH_BAD_ARG;
+    }
+    HostMeshU m;
+    try {
+        m = build_mesh_u(lists);
+    } catch (const std::exception &) {
+        return LULESH_BAD_ARG;
+    }
+    lulesh_domain *d = new (std::nothrow) lulesh_domain();
+    if (!d) {
+        return LULESH_BAD_ARG;
+    }
+    d->cfg = c;
+    d->unstructured = true;
+    d->nx = 0;
+    d->num_elem = m.num_elem;
+    d->num_node = m.num_node;
+    d->num_reg = 1;
+    d->reg_num_list.assign(size_t(m.num_elem), 1);
+    d->origin = m.origin;
+    d->prm = make_params(LULESH_IC_1_0);
+    if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete d;
+        return LULESH_HIP_ERROR;
+    }
+    if (hipEventCreate(&d->step_begin) != hipSuccess ||
+        hipEventCreate(&d->step_end) != hipSuccess) {
+        lulesh_destroy(d);
+        return LULESH_HIP_ERROR;
+    }
+    const int rc = build_device_domain_u(d, m);
+    if (rc != LULESH_OK) {
+        lulesh_destroy(d);
+        return rc;
+    }
+    *out = d;
+    return LULESH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lulesh_create_lmesh(const char *path, const lulesh_config *cfg, lulesh_domain **out)
+{
+    if (!path || !out) {
+        return LULESH_BAD_ARG;
+    }
+    MeshLists l;
+    try {
+        l = read_lmesh(path);
+    } catch (const std::exception &) {
+        return LULESH_BAD_ARG;
+    }
+    return create_domain_u(l, cfg, out);
+}
+
+int lulesh_create_mesh(int num_elem, int num_node, const double *x, const double *y,
+                       const double *z, const int *nodelist, const int *neighbors,
+                       int nsx, const int *symm_x, int nsy, const int *symm_y, int nsz,
+                       const int *symm_z, int nfree, const int *free_surf,
+                       const lulesh_config *cfg, lulesh_domain **out)
+{
+    if (num_elem < 1 || num_node < 8 || !x || !y || !z || !nodelist || !neighbors ||
+        nsx < 0 || nsy < 0 || nsz < 0 || nfree < 0 || (nsx && !symm_x) ||
+        (nsy && !symm_y) || (nsz && !symm_z) || (nfree && !free_surf)) {
+        return LULESH_BAD_ARG;
+    }
+    MeshLists l;
+    l.num_elem = num_elem;
+    l.num_node = num_node;
+    l.x.assign(x, x + num_node);
+    l.y.assign(y, y + num_node);
+    l.z.assign(z, z + num_node);
+    l.nodelist.assign(nodelist, nodelist + size_t(num_elem) * 8);
+    l.neighbors.assign(neighbors, neighbors + size_t(num_elem) * 6);
+    if (nsx) l.symm_x.assign(symm_x, symm_x + nsx);
+    if (nsy) l.symm_y.assign(symm_y, symm_y + nsy);
+    if (nsz) l.symm_z.assign(symm_z, symm_z + nsz);
+    if (nfree) l.free_surf.assign(free_surf, free_surf + nfree);
+    return create_domain_u(l, cfg, out);
+}
+
+int lulesh_origin_element(const lulesh_domain *d) { return d ? d->origin : -1; }
+
+int lulesh_get_nodelist(lulesh_domain *d, int *nodelist)
+{
+    if (!d || !nodelist) {
+        return LULESH_BAD_ARG;
+    }
+    const int64_t ne = d->num_elem;
+    if (!d->unstructured) {
+        // the cube's corners: bottom face counter-clockwise, then the top face
+        const int n = d->nx, en = n + 1, pl = en * en;
+        int64_t el = 0;
+        for (int k = 0; k < n; ++k) {
+            for (int j = 0; j < n; ++j) {
+                for (int i = 0; i < n; ++i, ++el) {
+                    const int n0 = (k * en + j) * en + i;
+                    const int c[8] = {n0, n0 + 1, n0 + en + 1, n0 + en,
+                                      n0 + pl, n0 + pl + 1, n0 + pl + en + 1, n0 + pl + en};
+                    for (int q = 0; q < 8; ++q) nodelist[el * 8 + q] = c[q];
+                }
+            }
+        }
+        return LULESH_OK;
+    }
+    std::vector<int32_t> cm(size_t(8 * ne));  // corner-major on the device
+    HIP_TRY(d, hipMemcpyAsync(cm.data(), d->mesh.nodelist, cm.size() * sizeof(int32_t),
+                              hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(d, hipStreamSynchronize(d->stream));
+    for (int64_t el = 0; el < ne; ++el) {
+        for (int q = 0; q < 8; ++q) nodelist[el * 8 + q] = cm[size_t(q * ne + el)];
+    }
+    return LULESH_OK;
+}
+
 void lulesh_destroy(lulesh_domain *d)
 {
     if (!d) {
@@ -407,6 +667,7 @@ void lulesh_destroy(lulesh_domain *d)
     if (d->step_begin) (void)hipEventDestroy(d->step_begin);
     if (d->step_end) (void)hipEventDestroy(d->step_end);
     if (d->arena) (void)hipFree(d->arena);
+    for (void *b : d->bufs) (void)hipFree(b);
     if (d->stream) (void)hipStreamDestroy(d->stream);
     delete d;
 }
@@ -556,7 +817,8 @@ int lulesh_get_stats(lulesh_domain *d, lulesh_stats *out)
     int rc = read_scalars(d, &s);
     if (rc) return rc;
     double e0 = 0.0;
-    HIP_TRY(d, hipMemcpy(&e0, energy_ptr(d), sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(d, hipMemcpy(&e0, energy_ptr(d) + d->origin, sizeof(double),
+                         hipMemcpyDeviceToHost));
     out->time = s.time;
     out->deltatime = s.deltatime;
     out->dtcourant = s.dtcourant;
@@ -633,6 +895,9 @@ int lulesh_set_sweep_shape(lulesh_domain *d, int tx, int ty, int nz, int waves)
     if (!d) {
         return LULESH_BAD_ARG;
     }
+    if (d->unstructured) {
+        return fail(d, LULESH_BAD_ARG, "an unstructured mesh has no sweep");
+    }
     const SweepShape sh = {tx, ty, nz, waves};
     if (!(d->mixed ? force_sweep_mixed_shape_supported(sh)
                    : force_sweep_shape_supported(sh))) {
@@ -669,6 +934,9 @@ int lulesh_symmetry(lulesh_domain *d, double *max_abs_diff,
 {
     if (!d) {
         return LULESH_BAD_ARG;
+    }
+    if (d->unstructured) {
+        return fail(d, LULESH_BAD_ARG, "the plane-0 symmetry check needs the cube");
     }
     const int nx = d->nx;
     std::vector<double> e(size_t(nx) * nx);
@@ -729,8 +997,9 @@ int lulesh_set_learned_q(lulesh_domain *d, const lulesh_mlp *net, double z_max)
     if (!d || (net && !(z_max > 0.0))) {
         return LULESH_BAD_ARG;
     }
-    if (net && d->mixed) {
-        return fail(d, LULESH_BAD_ARG, "the learned closure runs on float64 domains");
+    if (net && (d->mixed || d->unstructured)) {
+        return fail(d, LULESH_BAD_ARG,
+                    "the learned closure runs on float64 structured domains");
     }
     d->learned = net ? lulesh_mlp_device_frags(net) : nullptr;
     d->learned_z_max = float(z_max);
@@ -742,8 +1011,8 @@ int lulesh_q_features(lulesh_domain *d, float *features, double *q)
     if (!d || !features || !q) {
         return LULESH_BAD_ARG;
     }
-    if (d->mixed) {
-        return fail(d, LULESH_BAD_ARG, "features are formed on float64 domains");
+    if (d->mixed || d->unstructured) {
+        return fail(d, LULESH_BAD_ARG, "features are formed on float64 structured domains");
     }
     // kinematics and gradients of the current state (dt = 0: the half-step
     // geometry is the current one) under a temporarily active cycle

@@ -390,6 +390,177 @@ hipError_t launch_shape_batch(int nz, int blocks, const BatchMember *members,
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Unstructured meshes.  With an arbitrary connectivity a node's elements are
+// not a fixed stencil, so the element phase writes its 8 corner forces
+// (stress and hourglass apart) and the node phase gathers each node's
+// corners in ascending element order from a corner list: the reference's
+// own scheme (reference/C/LULESH-OMP.cc:853-931, :1475-1496, :209-268).  The
+// element arithmetic is the sweep's, expression for expression.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_corner_forces_u(DevFields f, DevMeshU m,
+                                                         DevScalars *s, PhysParams prm)
+{
+    if (s->active == 0 || s->error != 0) {
+        return;
+    }
+    const int el = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
+    const int ne = f.num_elem;
+    if (el >= ne) {
+        return;
+    }
+    const int par = s->cur;
+    int nodes[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        nodes[c] = m.nodelist[c * ne + el];
+    }
+    double x[8], y[8], z[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        x[c] = f.x[par][nodes[c]];
+        y[c] = f.y[par][nodes[c]];
+        z[c] = f.z[par][nodes[c]];
+    }
+    {
+        const double sig = -f.p[el] - f.q[el];
+        const double det = jacobian_volume(x, y, z);
+        if (det <= 0.0) {
+            atomicCAS(&s->error, 0, -1);
+        }
+        double pf[3][8];
+        node_normals(pf[0], pf[1], pf[2], x, y, z);
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                m.fs[t][c * ne + el] = -(sig * pf[t][c]);
+            }
+        }
+    }
+    if (!(prm.hgcoef > 0.0)) {
+        return;
+    }
+    const double vv = f.v[el];
+    if (vv <= 0.0) {
+        atomicCAS(&s->error, 0, -1);
+    }
+    const double determ = f.volo[el] * vv;
+    const double volinv = 1.0 / determ;
+    double hourgam[8][4];
+    {
+        double hm[4][3];
+#pragma unroll
+        for (int i1 = 0; i1 < 4; ++i1) {
+            double hmx = x[0] * fb_gamma(i1, 0);
+            double hmy = y[0] * fb_gamma(i1, 0);
+            double hmz = z[0] * fb_gamma(i1, 0);
+#pragma unroll
+            for (int n = 1; n < 8; ++n) {
+                hmx = hmx + x[n] * fb_gamma(i1, n);
+                hmy = hmy + y[n] * fb_gamma(i1, n);
+                hmz = hmz + z[n] * fb_gamma(i1, n);
+            }
+            hm[i1][0] = hmx;
+            hm[i1][1] = hmy;
+            hm[i1][2] = hmz;
+        }
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            const int *st = kStencil[n];
+            double dvx, dvy, dvz;
+            volu_der(x, y, z, st[0], st[1], st[2], st[3], st[4], st[5], dvx, dvy, dvz);
+#pragma unroll
+            for (int i1 = 0; i1 < 4; ++i1) {
+                hourgam[n][i1] = fb_gamma(i1, n) -
+                                 volinv * (dvx * hm[i1][0] + dvy * hm[i1][1] + dvz * hm[i1][2]);
+            }
+        }
+    }
+    const double volume13 = lulesh_cbrt(determ);
+    const double coefficient = -prm.hgcoef * 0.01 * f.ss[el] * f.elem_mass[el] / volume13;
+    const double *vel[3] = {f.xd[par], f.yd[par], f.zd[par]};
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        double u[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            u[c] = vel[t][nodes[c]];
+        }
+        double h[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            h[i] = hourgam[0][i] * u[0] + hourgam[1][i] * u[1] + hourgam[2][i] * u[2] +
+                   hourgam[3][i] * u[3] + hourgam[4][i] * u[4] + hourgam[5][i] * u[5] +
+                   hourgam[6][i] * u[6] + hourgam[7][i] * u[7];
+        }
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            m.fh[t][n * ne + el] =
+                coefficient * (hourgam[n][0] * h[0] + hourgam[n][1] * h[1] +
+                               hourgam[n][2] * h[2] + hourgam[n][3] * h[3]);
+        }
+    }
+}
+
+// gather, acceleration, symmetry planes, velocity and position of each node
+// (:905-926, :1475-1496, :1613-1683); positions and velocities are updated
+// in place (the corner forces are complete before this kernel runs)
+__global__ __launch_bounds__(256) void k_nodes_u(DevFields f, DevMeshU m, DevScalars *s,
+                                                 PhysParams prm)
+{
+    if (s->active == 0 || s->error != 0) {
+        return;
+    }
+    const int n = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
+    if (n >= f.num_node) {
+        return;
+    }
+    const int par = s->cur;
+    const double dt = s->deltatime;
+    const int b = m.corner_start[n], e = m.corner_start[n + 1];
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int i = b; i < e; ++i) {
+        const int c = m.corner_list[i];
+        sx += m.fs[0][c];
+        sy += m.fs[1][c];
+        sz += m.fs[2][c];
+    }
+    double fx = sx, fy = sy, fz = sz;
+    if (prm.hgcoef > 0.0) {
+        double hx = 0.0, hy = 0.0, hz = 0.0;
+        for (int i = b; i < e; ++i) {
+            const int c = m.corner_list[i];
+            hx += m.fh[0][c];
+            hy += m.fh[1][c];
+            hz += m.fh[2][c];
+        }
+        fx += hx;
+        fy += hy;
+        fz += hz;
+    }
+    const double mass = f.nodal_mass[n];
+    double xdd = fx / mass;
+    double ydd = fy / mass;
+    double zdd = fz / mass;
+    const int sym = m.symm[n];
+    if (sym & 1) xdd = 0.0;
+    if (sym & 2) ydd = 0.0;
+    if (sym & 4) zdd = 0.0;
+    double xd = f.xd[par][n] + xdd * dt;
+    if (fabs(xd) < prm.u_cut) xd = 0.0;
+    double yd = f.yd[par][n] + ydd * dt;
+    if (fabs(yd) < prm.u_cut) yd = 0.0;
+    double zd = f.zd[par][n] + zdd * dt;
+    if (fabs(zd) < prm.u_cut) zd = 0.0;
+    f.xd[par][n] = xd;
+    f.yd[par][n] = yd;
+    f.zd[par][n] = zd;
+    f.x[par][n] = f.x[par][n] + xd * dt;
+    f.y[par][n] = f.y[par][n] + yd * dt;
+    f.z[par][n] = f.z[par][n] + zd * dt;
+}
+
 // (TX, TY, minimum waves per SIMD the register budget must allow); the
 // number of node planes per block is a launch argument
 #define LULESH_SWEEP_SHAPES(X) \

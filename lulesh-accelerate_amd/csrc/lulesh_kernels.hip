@@ -162,17 +162,12 @@ __global__ void k_time_increment_batch(const BatchMember *__restrict__ members,
 // ---------------------------------------------------------------------------
 namespace {
 
-__device__ __forceinline__ void kinematics_q(const DevFields &f, DevScalars *s, int el)
+// the element's eight nodes are given (structured or from a node list)
+__device__ __forceinline__ void kinematics_q_nodes(const DevFields &f, DevScalars *s,
+                                                   int el, const int (&nodes)[8])
 {
     const int par = s->cur;
     const double dt = s->deltatime;
-    int i, j, k;
-    elem_ijk(f, el, i, j, k);
-    const int en = f.nx + 1;
-    const int n0 = (k * en + j) * en + i;
-    const int pl = en * en;
-    const int nodes[8] = {n0,      n0 + 1,      n0 + en + 1,      n0 + en,
-                          n0 + pl, n0 + pl + 1, n0 + pl + en + 1, n0 + pl + en};
     const double *__restrict__ px = f.x[par];
     const double *__restrict__ py = f.y[par];
     const double *__restrict__ pz = f.z[par];
@@ -270,6 +265,18 @@ __device__ __forceinline__ void kinematics_q(const DevFields &f, DevScalars *s, 
     dyv = -0.25 * ((yv[0] + yv[1] + yv[5] + yv[4]) - (yv[3] + yv[2] + yv[6] + yv[7]));
     dzv = -0.25 * ((zv[0] + zv[1] + zv[5] + zv[4]) - (zv[3] + zv[2] + zv[6] + zv[7]));
     f.delv_eta[el] = ax * dxv + ay * dyv + az * dzv;
+}
+
+__device__ __forceinline__ void kinematics_q(const DevFields &f, DevScalars *s, int el)
+{
+    int i, j, k;
+    elem_ijk(f, el, i, j, k);
+    const int en = f.nx + 1;
+    const int n0 = (k * en + j) * en + i;
+    const int pl = en * en;
+    const int nodes[8] = {n0,      n0 + 1,      n0 + en + 1,      n0 + en,
+                          n0 + pl, n0 + pl + 1, n0 + pl + en + 1, n0 + pl + en};
+    kinematics_q_nodes(f, s, el, nodes);
 }
 
 }  // namespace
@@ -376,6 +383,10 @@ struct QInputs {
     double vdov, vnew, volo;
 };
 
+// the rest of the inputs once the limiter values are known
+__device__ __forceinline__ void q_inputs_rest(const DevFields &f, int el, double dvxi,
+                                              double dveta, double dvzeta, QInputs &in);
+
 __device__ __forceinline__ void q_inputs(const DevFields &f, const PhysParams &prm,
                                          int el, int i, int j, int k, QInputs &in)
 {
@@ -389,6 +400,45 @@ __device__ __forceinline__ void q_inputs(const DevFields &f, const PhysParams &p
                          (j == nx - 1) ? 0.0 : f.delv_eta[el + sy], lm, ms);
     in.phi[2] = mono_phi(dvzeta, (k == 0) ? dvzeta : f.delv_zeta[el - sz],
                          (k == nx - 1) ? 0.0 : f.delv_zeta[el + sz], lm, ms);
+    q_inputs_rest(f, el, dvxi, dveta, dvzeta, in);
+}
+
+// the neighbour value across one face of an unstructured element: own value
+// on a symmetry face, 0 on a free face, else the neighbour's
+// (reference/C/LULESH-OMP.cc:2201-2287)
+__device__ __forceinline__ double face_value(const double *a, int el, int nbr, int bc,
+                                             int symm, int freeb)
+{
+    if (bc & symm) return a[el];
+    if (bc & freeb) return 0.0;
+    return a[nbr];
+}
+
+__device__ __forceinline__ void q_inputs_u(const DevFields &f, const DevMeshU &m,
+                                           const PhysParams &prm, int el, QInputs &in)
+{
+    const int ne = f.num_elem;
+    const int bc = m.elem_bc[el];
+    const double dvxi = f.delv_xi[el], dveta = f.delv_eta[el], dvzeta = f.delv_zeta[el];
+    const double lm = prm.monoq_limiter_mult, ms = prm.monoq_max_slope;
+    in.phi[0] = mono_phi(dvxi,
+                         face_value(f.delv_xi, el, m.nbr[0 * ne + el], bc, kBcXiMSymm, kBcXiMFree),
+                         face_value(f.delv_xi, el, m.nbr[1 * ne + el], bc, kBcXiPSymm, kBcXiPFree),
+                         lm, ms);
+    in.phi[1] = mono_phi(dveta,
+                         face_value(f.delv_eta, el, m.nbr[2 * ne + el], bc, kBcEtaMSymm, kBcEtaMFree),
+                         face_value(f.delv_eta, el, m.nbr[3 * ne + el], bc, kBcEtaPSymm, kBcEtaPFree),
+                         lm, ms);
+    in.phi[2] = mono_phi(dvzeta,
+                         face_value(f.delv_zeta, el, m.nbr[4 * ne + el], bc, kBcZetaMSymm, kBcZetaMFree),
+                         face_value(f.delv_zeta, el, m.nbr[5 * ne + el], bc, kBcZetaPSymm, kBcZetaPFree),
+                         lm, ms);
+    q_inputs_rest(f, el, dvxi, dveta, dvzeta, in);
+}
+
+__device__ __forceinline__ void q_inputs_rest(const DevFields &f, int el, double dvxi,
+                                              double dveta, double dvzeta, QInputs &in)
+{
     in.vdov = f.vdov[el];
     in.vnew = f.vnew[el];
     in.volo = f.volo[el];
@@ -748,6 +798,54 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_learned(DevFields f, DevScalar
     }
     unsigned long long cand_c = as_bits(kBig), cand_h = as_bits(kBig);
     if (valid) {
+        eos_dt_elem(f, s, prm, el, rep, in.vdov, in.vnew, qlin, qquad, cand_c, cand_h);
+    }
+    fold_minima(s, cand_c, cand_h, lds_c, lds_h);
+}
+
+// ---------------------------------------------------------------------------
+// Unstructured meshes: the same element phases, nodes and face neighbours
+// from the mesh's lists
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_kinematics_q_u(DevFields f, DevMeshU m,
+                                                        DevScalars *s, PhysParams prm)
+{
+    if (s->active == 0 || s->error != 0) {
+        return;
+    }
+    const int el = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
+    const int ne = f.num_elem;
+    if (el < ne) {
+        int nodes[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            nodes[c] = m.nodelist[c * ne + el];
+        }
+        kinematics_q_nodes(f, s, el, nodes);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_q_eos_dt_u(DevFields f, DevMeshU m, DevScalars *s,
+                                                    PhysParams prm)
+{
+    __shared__ unsigned long long lds_c[4], lds_h[4];
+    __shared__ int skip;
+    if (threadIdx.x == 0) {
+        skip = (s->active == 0 || s->error != 0) ? 1 : 0;
+    }
+    __syncthreads();
+    if (skip) {
+        return;
+    }
+    const int slot = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
+    unsigned long long cand_c = as_bits(kBig), cand_h = as_bits(kBig);
+    if (slot < f.num_elem) {
+        const int el = f.eos_order[slot];
+        const int rep = f.eos_rep[slot];
+        QInputs in;
+        q_inputs_u(f, m, prm, el, in);
+        double qlin, qquad;
+        monotonic_q(prm, in, qlin, qquad);
         eos_dt_elem(f, s, prm, el, rep, in.vdov, in.vnew, qlin, qquad, cand_c, cand_h);
     }
     fold_minima(s, cand_c, cand_h, lds_c, lds_h);

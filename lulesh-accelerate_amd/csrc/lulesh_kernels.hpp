@@ -165,6 +165,31 @@ hipError_t launch_force_sweep_batch(const SweepShape &shape, int blocks,
                                     const PhysParams &prm, hipStream_t stream);
 bool force_sweep_shape_supported(const SweepShape &shape);
 
+// Unstructured hexahedral meshes (the .lmesh input of the reference's CUDA
+// port): explicit connectivity and boundary masks.  Corner-major layouts
+// ([c][num_elem]) keep a wavefront's accesses contiguous.
+struct DevMeshU {
+    const int32_t *nodelist;      // [8][num_elem]
+    const int32_t *nbr;           // [6][num_elem]: xi-, xi+, eta-, eta+, zeta-, zeta+
+    const int32_t *elem_bc;       // [num_elem] face masks (kBc*)
+    const int32_t *corner_start;  // [num_node + 1]
+    const int32_t *corner_list;   // corners c * num_elem + el of each node, by element
+    const uint8_t *symm;          // [num_node] bit 0 / 1 / 2: x / y / z symmetry plane
+    double *fs[3];                // stress corner forces [8][num_elem], per component
+    double *fh[3];                // hourglass corner forces
+};
+
+// element face boundary masks (reference/C/LULESH-OMP.cc:532-554)
+constexpr int kBcXiMSymm = 0x001, kBcXiMFree = 0x002, kBcXiPSymm = 0x004,
+              kBcXiPFree = 0x008, kBcEtaMSymm = 0x010, kBcEtaMFree = 0x020,
+              kBcEtaPSymm = 0x040, kBcEtaPFree = 0x080, kBcZetaMSymm = 0x100,
+              kBcZetaMFree = 0x200, kBcZetaPSymm = 0x400, kBcZetaPFree = 0x800;
+
+__global__ void k_corner_forces_u(DevFields f, DevMeshU m, DevScalars *s, PhysParams prm);
+__global__ void k_nodes_u(DevFields f, DevMeshU m, DevScalars *s, PhysParams prm);
+__global__ void k_kinematics_q_u(DevFields f, DevMeshU m, DevScalars *s, PhysParams prm);
+__global__ void k_q_eos_dt_u(DevFields f, DevMeshU m, DevScalars *s, PhysParams prm);
+
 // The learned closure (network in lulesh_mlp.hpp).
 struct MlpFrags;
 __global__ void k_q_features(DevFields f, PhysParams prm, float *feat, double *q);

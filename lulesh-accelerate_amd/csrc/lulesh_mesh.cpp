@@ -5,8 +5,12 @@
 // bit-identical to the reference's initial state.
 #include "lulesh_mesh.hpp"
 
+#include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <stdexcept>
+#include <string>
 
 namespace lulesh {
 
@@ -265,6 +269,177 @@ HostMesh build_mesh(const MeshConfig &cfg)
         m.eos_rep[static_cast<size_t>(slot)] = static_cast<uint16_t>(
             region_rep(cfg.ic_mode, m.num_reg, cfg.cost, r));
     }
+    return m;
+}
+
+MeshLists read_lmesh(const char *path)
+{
+    std::FILE *fp = std::fopen(path, "rb");
+    if (!fp) {
+        throw std::invalid_argument(std::string("cannot open ") + path);
+    }
+    std::string text;
+    char buf[1 << 16];
+    size_t got;
+    while ((got = std::fread(buf, 1, sizeof buf, fp)) > 0) {
+        text.append(buf, got);
+    }
+    std::fclose(fp);
+    const char *p = text.c_str();
+    const char *end = p + text.size();
+    auto next_int = [&](const char *what) -> long {
+        char *q = nullptr;
+        const long v = std::strtol(p, &q, 10);
+        if (q == p || q > end) {
+            throw std::invalid_argument(std::string("lmesh: expected ") + what);
+        }
+        p = q;
+        return v;
+    };
+    auto next_double = [&]() -> double {
+        char *q = nullptr;
+        const double v = std::strtod(p, &q);
+        if (q == p || q > end) {
+            throw std::invalid_argument("lmesh: expected a coordinate");
+        }
+        p = q;
+        return v;
+    };
+    MeshLists l;
+    const long ne = next_int("the element count"), nn = next_int("the node count");
+    if (ne < 1 || nn < 8 || ne > (1L << 26) || nn > (1L << 27)) {
+        throw std::invalid_argument("lmesh: element / node counts out of range");
+    }
+    l.num_elem = int(ne);
+    l.num_node = int(nn);
+    l.x.resize(size_t(nn));
+    l.y.resize(size_t(nn));
+    l.z.resize(size_t(nn));
+    for (long i = 0; i < nn; ++i) {
+        l.x[size_t(i)] = next_double();
+        l.y[size_t(i)] = next_double();
+        l.z[size_t(i)] = next_double();
+    }
+    l.nodelist.resize(size_t(ne) * 8);
+    for (auto &v : l.nodelist) v = int32_t(next_int("a node index"));
+    l.neighbors.resize(size_t(ne) * 6);
+    for (auto &v : l.neighbors) v = int32_t(next_int("a neighbour index"));
+    std::vector<int32_t> *sets[4] = {&l.symm_x, &l.symm_y, &l.symm_z, &l.free_surf};
+    for (auto *set : sets) {
+        const long n = next_int("a node set size");
+        if (n < 0 || n > nn) {
+            throw std::invalid_argument("lmesh: node set size out of range");
+        }
+        set->resize(size_t(n));
+        for (auto &v : *set) v = int32_t(next_int("a node set entry"));
+    }
+    return l;
+}
+
+HostMeshU build_mesh_u(const MeshLists &l)
+{
+    const int ne = l.num_elem, nn = l.num_node;
+    if (ne < 1 || nn < 8 || l.x.size() != size_t(nn) || l.y.size() != size_t(nn) ||
+        l.z.size() != size_t(nn) || l.nodelist.size() != size_t(ne) * 8 ||
+        l.neighbors.size() != size_t(ne) * 6) {
+        throw std::invalid_argument("mesh: array sizes do not match the counts");
+    }
+    for (int32_t v : l.nodelist) {
+        if (v < 0 || v >= nn) throw std::invalid_argument("mesh: node index out of range");
+    }
+    for (int32_t v : l.neighbors) {
+        if (v < 0 || v >= ne) throw std::invalid_argument("mesh: neighbour out of range");
+    }
+    for (const auto *set : {&l.symm_x, &l.symm_y, &l.symm_z, &l.free_surf}) {
+        for (int32_t v : *set) {
+            if (v < 0 || v >= nn) throw std::invalid_argument("mesh: node set entry out of range");
+        }
+    }
+    HostMeshU m;
+    m.num_elem = ne;
+    m.num_node = nn;
+    m.x = l.x;
+    m.y = l.y;
+    m.z = l.z;
+    m.nodelist.resize(size_t(ne) * 8);
+    m.nbr.resize(size_t(ne) * 6);
+    for (int el = 0; el < ne; ++el) {
+        for (int c = 0; c < 8; ++c) m.nodelist[size_t(c) * ne + el] = l.nodelist[size_t(el) * 8 + c];
+        for (int d = 0; d < 6; ++d) m.nbr[size_t(d) * ne + el] = l.neighbors[size_t(el) * 6 + d];
+    }
+    // symmetry planes of the nodes
+    m.symm.assign(size_t(nn), 0);
+    for (int32_t v : l.symm_x) m.symm[size_t(v)] |= 1;
+    for (int32_t v : l.symm_y) m.symm[size_t(v)] |= 2;
+    for (int32_t v : l.symm_z) m.symm[size_t(v)] |= 4;
+    // face masks: a face with its four nodes on the symmetry sets, then (as
+    // the reference's reader marks them) a face with its four nodes on the
+    // free-surface set, is a symmetry face (lulesh.cu:852-913)
+    constexpr int kZM = 0x100, kZP = 0x400, kEM = 0x010, kEP = 0x040, kXM = 0x001, kXP = 0x004;
+    m.elem_bc.assign(size_t(ne), 0);
+    std::vector<int> surf(static_cast<size_t>(nn));
+    for (int pass = 0; pass < 2; ++pass) {
+        std::fill(surf.begin(), surf.end(), 0);
+        if (pass == 0) {
+            for (const auto *set : {&l.symm_x, &l.symm_y, &l.symm_z})
+                for (int32_t v : *set) surf[size_t(v)] = 1;
+        } else {
+            for (int32_t v : l.free_surf) surf[size_t(v)] = 1;
+        }
+        for (int el = 0; el < ne; ++el) {
+            int mask = 0;
+            for (int c = 0; c < 8; ++c) mask |= surf[size_t(l.nodelist[size_t(el) * 8 + c])] << c;
+            int32_t &bc = m.elem_bc[size_t(el)];
+            if ((mask & 0x0f) == 0x0f) bc |= kZM;
+            if ((mask & 0xf0) == 0xf0) bc |= kZP;
+            if ((mask & 0x33) == 0x33) bc |= kEM;
+            if ((mask & 0xcc) == 0xcc) bc |= kEP;
+            if ((mask & 0x99) == 0x99) bc |= kXM;
+            if ((mask & 0x66) == 0x66) bc |= kXP;
+        }
+        if (pass == 0) {
+            m.origin = 0;
+            for (int el = 0; el < ne; ++el) {
+                if (m.elem_bc[size_t(el)] == (kXM | kEM | kZM)) {
+                    m.origin = el;
+                    break;
+                }
+            }
+        }
+    }
+    // node -> corners in ascending element order
+    m.corner_start.assign(size_t(nn) + 1, 0);
+    for (int32_t v : l.nodelist) ++m.corner_start[size_t(v) + 1];
+    for (int n = 0; n < nn; ++n) m.corner_start[size_t(n) + 1] += m.corner_start[size_t(n)];
+    m.corner_list.resize(size_t(ne) * 8);
+    {
+        std::vector<int32_t> fill(m.corner_start.begin(), m.corner_start.end() - 1);
+        for (int el = 0; el < ne; ++el) {
+            for (int c = 0; c < 8; ++c) {
+                const int32_t n = l.nodelist[size_t(el) * 8 + c];
+                m.corner_list[size_t(fill[size_t(n)]++)] = int32_t(c * ne + el);
+            }
+        }
+    }
+    // volumes and masses, summed into the nodes in element order
+    m.volo.resize(size_t(ne));
+    m.nodal_mass.assign(size_t(nn), 0.0);
+    for (int el = 0; el < ne; ++el) {
+        double xl[8], yl[8], zl[8];
+        for (int c = 0; c < 8; ++c) {
+            const int32_t n = l.nodelist[size_t(el) * 8 + c];
+            xl[c] = l.x[size_t(n)];
+            yl[c] = l.y[size_t(n)];
+            zl[c] = l.z[size_t(n)];
+        }
+        const double vol = hex_volume(xl, yl, zl);
+        m.volo[size_t(el)] = vol;
+        for (int c = 0; c < 8; ++c) {
+            m.nodal_mass[size_t(l.nodelist[size_t(el) * 8 + c])] += vol / 8.0;
+        }
+    }
+    m.e.assign(size_t(ne), 0.0);
+    m.e[size_t(m.origin)] = 3.948746e+7;
     return m;
 }
 

@@ -43,6 +43,41 @@ struct HostMesh {
 // Builds the mesh, masses, ICs and regions; throws std::invalid_argument.
 HostMesh build_mesh(const MeshConfig &cfg);
 
+// An unstructured hexahedral mesh in LULESH 1.0 semantics, set up as the
+// reference's .lmesh reader sets it up (reference/CUDA/
+// lulesh-kepler-singlegpu/lulesh.cu:728-925): element face masks derived
+// from the node sets, the energy in the octant corner, masses from the
+// element volumes.  Arrays the GPU reads are corner-major ([c][num_elem]).
+struct HostMeshU {
+    int num_elem = 0, num_node = 0;
+    std::vector<double> x, y, z, nodal_mass, volo, e;
+    std::vector<int32_t> nodelist;      // [8][num_elem]
+    std::vector<int32_t> nbr;           // [6][num_elem]
+    std::vector<int32_t> elem_bc;       // face masks
+    std::vector<int32_t> corner_start;  // [num_node + 1]
+    std::vector<int32_t> corner_list;   // c * num_elem + el, ascending el per node
+    std::vector<uint8_t> symm;          // bit 0 / 1 / 2: on the x / y / z symmetry set
+    int origin = 0;                     // element holding the initial energy
+};
+
+// The raw contents of a mesh: nodelist [num_elem][8] and neighbors
+// [num_elem][6] element-major, as the .lmesh file lists them.
+struct MeshLists {
+    int num_elem = 0, num_node = 0;
+    std::vector<double> x, y, z;
+    std::vector<int32_t> nodelist, neighbors;
+    std::vector<int32_t> symm_x, symm_y, symm_z, free_surf;
+};
+
+// Reads a .lmesh file (the format of lulesh.cu:728-848: counts, node
+// coordinates, 8 nodes per element, 6 face neighbours per element, the x, y,
+// z symmetry node sets and the free-surface node set, each set preceded by
+// its size); throws std::invalid_argument on a malformed file.
+MeshLists read_lmesh(const char *path);
+
+// Validates the lists and builds the mesh; throws std::invalid_argument.
+HostMeshU build_mesh_u(const MeshLists &l);
+
 // Number of EOS evaluations for region r (lulesh-2.0.3/lulesh.cc:2413-2424).
 int region_rep(int ic_mode, int num_reg, int cost, int r);
 

@@ -7,7 +7,8 @@
 // read: an unstructured grid of hexahedra with the node velocities as a
 // point vector field and e, p, v, q as cell fields, plus the time and cycle.
 // LULESH's corner order (bottom face counter-clockwise, then the top face)
-// is VTK's hexahedron order, so the connectivity is written as it is.
+// is VTK's hexahedron order, so the connectivity (the cube's, or an
+// unstructured mesh's own lists) is written as it is.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -41,29 +42,18 @@ int edge_of(int num_elem)
     return -1;
 }
 
-}  // namespace
-
-extern "C" {
-
-int lulesh_write_domain_arrays(const char *path, int num_elem, int step, double time,
-                               const double *x, const double *y, const double *z,
-                               const double *xd, const double *yd, const double *zd,
-                               const double *e, const double *p, const double *v,
-                               const double *q)
+// Legacy binary VTK of num_elem hexahedra over num_node nodes; nodelist
+// holds 8 node indices per element in LULESH's corner order.
+int write_vtk(const char *path, int num_elem, int64_t num_node, const int *nodelist,
+              int step, double time, const double *x, const double *y, const double *z,
+              const double *xd, const double *yd, const double *zd, const double *e,
+              const double *p, const double *v, const double *q)
 {
-    if (!path || !x || !y || !z || !xd || !yd || !zd || !e || !p || !v || !q) {
-        return LULESH_BAD_ARG;
-    }
-    const int n = edge_of(num_elem);
-    if (n < 1) {
-        return LULESH_BAD_ARG;  // the structured cube: num_elem = n^3
-    }
-    const int en = n + 1;
-    const int64_t nn = int64_t(en) * en * en;
     std::FILE *fp = std::fopen(path, "wb");
     if (!fp) {
         return LULESH_BAD_ARG;
     }
+    const int64_t nn = num_node;
     std::vector<unsigned char> buf;
     auto text = [&](const std::string &s) { buf.insert(buf.end(), s.begin(), s.end()); };
     char line[256];
@@ -85,17 +75,9 @@ int lulesh_write_domain_arrays(const char *path, int num_elem, int step, double 
     std::snprintf(line, sizeof line, "\nCELLS %d %lld\n", num_elem,
                   static_cast<long long>(num_elem) * 9);
     text(line);
-    for (int k = 0; k < n; ++k) {
-        for (int j = 0; j < n; ++j) {
-            for (int i = 0; i < n; ++i) {
-                const uint32_t n0 = uint32_t((k * en + j) * en + i);
-                const uint32_t pl = uint32_t(en * en);
-                const uint32_t c[8] = {n0, n0 + 1, n0 + en + 1, n0 + en,
-                                       n0 + pl, n0 + pl + 1, n0 + pl + en + 1, n0 + pl + en};
-                put_be32(buf, 8);
-                for (uint32_t cc : c) put_be32(buf, cc);
-            }
-        }
+    for (int64_t c = 0; c < num_elem; ++c) {
+        put_be32(buf, 8);
+        for (int k = 0; k < 8; ++k) put_be32(buf, uint32_t(nodelist[c * 8 + k]));
     }
     std::snprintf(line, sizeof line, "\nCELL_TYPES %d\n", num_elem);
     text(line);
@@ -124,6 +106,40 @@ int lulesh_write_domain_arrays(const char *path, int num_elem, int step, double 
     return (std::fclose(fp) == 0 && ok) ? LULESH_OK : LULESH_BAD_ARG;
 }
 
+}  // namespace
+
+extern "C" {
+
+int lulesh_write_domain_arrays(const char *path, int num_elem, int step, double time,
+                               const double *x, const double *y, const double *z,
+                               const double *xd, const double *yd, const double *zd,
+                               const double *e, const double *p, const double *v,
+                               const double *q)
+{
+    if (!path || !x || !y || !z || !xd || !yd || !zd || !e || !p || !v || !q) {
+        return LULESH_BAD_ARG;
+    }
+    const int n = edge_of(num_elem);
+    if (n < 1) {
+        return LULESH_BAD_ARG;  // the structured cube: num_elem = n^3
+    }
+    const int en = n + 1, pl = en * en;
+    std::vector<int> nodelist(size_t(num_elem) * 8);
+    size_t w = 0;
+    for (int k = 0; k < n; ++k) {
+        for (int j = 0; j < n; ++j) {
+            for (int i = 0; i < n; ++i) {
+                const int n0 = (k * en + j) * en + i;
+                const int c[8] = {n0, n0 + 1, n0 + en + 1, n0 + en,
+                                  n0 + pl, n0 + pl + 1, n0 + pl + en + 1, n0 + pl + en};
+                for (int cc : c) nodelist[w++] = cc;
+            }
+        }
+    }
+    return write_vtk(path, num_elem, int64_t(en) * en * en, nodelist.data(), step, time, x,
+                     y, z, xd, yd, zd, e, p, v, q);
+}
+
 int lulesh_write_domain(lulesh_domain *d, const char *path)
 {
     if (!d || !path) {
@@ -134,15 +150,16 @@ int lulesh_write_domain(lulesh_domain *d, const char *path)
     if (rc) return rc;
     const size_t nn = size_t(st.num_node), ne = size_t(st.num_elem);
     std::vector<double> node(6 * nn), elem(5 * ne);
+    std::vector<int> nodelist(8 * ne);
     double t = 0.0, dt = 0.0;
     rc = lulesh_download_state(d, &node[0], &node[nn], &node[2 * nn], &node[3 * nn],
                                &node[4 * nn], &node[5 * nn], &elem[0], &elem[ne],
                                &elem[2 * ne], &elem[3 * ne], &elem[4 * ne], &t, &dt);
     if (rc) return rc;
-    return lulesh_write_domain_arrays(path, st.num_elem, st.cycle, t, &node[0], &node[nn],
-                                      &node[2 * nn], &node[3 * nn], &node[4 * nn],
-                                      &node[5 * nn], &elem[0], &elem[ne], &elem[3 * ne],
-                                      &elem[2 * ne]);
+    if ((rc = lulesh_get_nodelist(d, nodelist.data()))) return rc;
+    return write_vtk(path, st.num_elem, int64_t(nn), nodelist.data(), st.cycle, t, &node[0],
+                     &node[nn], &node[2 * nn], &node[3 * nn], &node[4 * nn], &node[5 * nn],
+                     &elem[0], &elem[ne], &elem[3 * ne], &elem[2 * ne]);
 }
 
 }  // extern "C"

@@ -33,6 +33,8 @@ void usage(const char *prog)
         " -c <cost>       : Extra cost of more expensive regions (def: 1)\n"
         " -p              : Print out progress\n"
         " --ic <1|2>      : LULESH 1.0 or 2.0 initial conditions (def: 2)\n"
+        " -u <file>       : unstructured mesh from a .lmesh file (LULESH 1.0)\n"
+        " --vtk <file>    : write the final state as a VTK file\n"
         " -h              : This message\n",
         prog);
 }
@@ -61,6 +63,7 @@ int main(int argc, char **argv)
     lulesh_config cfg;
     lulesh_config_default(&cfg);
     bool quiet = false, progress = false;
+    std::string lmesh, vtk;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         if (a == "-s") {
@@ -75,6 +78,12 @@ int main(int argc, char **argv)
             if (!int_arg(argc, argv, i, cfg.cost)) return 1;
         } else if (a == "--ic") {
             if (!int_arg(argc, argv, i, cfg.ic_mode)) return 1;
+        } else if (a == "-u" || a == "--vtk") {
+            if (i + 1 >= argc) {
+                std::fprintf(stderr, "missing file argument to %s\n", argv[i]);
+                return 1;
+            }
+            (a == "-u" ? lmesh : vtk) = argv[++i];
         } else if (a == "-p") {
             progress = true;
         } else if (a == "-q") {
@@ -90,17 +99,28 @@ int main(int argc, char **argv)
     }
 
     lulesh_domain *d = nullptr;
-    int rc = lulesh_create(&cfg, &d);
+    int rc;
+    if (!lmesh.empty()) {
+        cfg.ic_mode = LULESH_IC_1_0;
+        rc = lulesh_create_lmesh(lmesh.c_str(), &cfg, &d);
+    } else {
+        rc = lulesh_create(&cfg, &d);
+    }
     if (rc != LULESH_OK) {
-        std::fprintf(stderr, "lulesh_create failed (%d)\n", rc);
+        std::fprintf(stderr, "creating the domain failed (%d)\n", rc);
         return 1;
     }
+    lulesh_stats st0;
+    lulesh_get_stats(d, &st0);
     if (!quiet) {
-        std::printf("Running problem size %d^3 per domain until completion\n", cfg.nx);
+        if (lmesh.empty()) {
+            std::printf("Running problem size %d^3 per domain until completion\n", cfg.nx);
+        } else {
+            std::printf("Running mesh %s until completion\n", lmesh.c_str());
+        }
         std::printf("Num processors: 1 (MI355X)\n");
         std::printf("Initial conditions: LULESH %s\n", cfg.ic_mode == 1 ? "1.0" : "2.0");
-        std::printf("Total number of elements: %lld\n\n",
-                    static_cast<long long>(cfg.nx) * cfg.nx * cfg.nx);
+        std::printf("Total number of elements: %d\n\n", st0.num_elem);
     }
 
     const auto t0 = std::chrono::steady_clock::now();
@@ -132,9 +152,14 @@ int main(int argc, char **argv)
     lulesh_stats st;
     lulesh_get_stats(d, &st);
     double mabs = 0, tabs = 0, mrel = 0;
-    lulesh_symmetry(d, &mabs, &tabs, &mrel);
+    lulesh_symmetry(d, &mabs, &tabs, &mrel);  // the cube only
+    int status = 0;
+    if (!vtk.empty() && lulesh_write_domain(d, vtk.c_str()) != LULESH_OK) {
+        std::fprintf(stderr, "writing %s failed\n", vtk.c_str());
+        status = 1;
+    }
     if (!quiet) {
-        const double zones = double(cfg.nx) * cfg.nx * cfg.nx;
+        const double zones = double(st.num_elem);
         const double grind = (elapsed * 1e6 / st.cycle) / zones;
         std::printf("Run completed:  \n");
         std::printf("   Problem size        =  %i \n", cfg.nx);
@@ -152,5 +177,5 @@ int main(int argc, char **argv)
         std::printf("zones*cycles/s       = %10.6e\n\n", zones * st.cycle / elapsed);
     }
     lulesh_destroy(d);
-    return 0;
+    return status;
 }

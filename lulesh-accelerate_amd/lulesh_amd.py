@@ -79,6 +79,8 @@ EXPORTS = (
     "lulesh_mlp_get_params", "lulesh_mlp_eval", "lulesh_mlp_gradient",
     "lulesh_mlp_train", "lulesh_mlp_last_error", "lulesh_set_learned_q",
     "lulesh_q_features", "lulesh_write_domain_arrays", "lulesh_write_domain",
+    "lulesh_create_lmesh", "lulesh_create_mesh", "lulesh_origin_element",
+    "lulesh_get_nodelist",
 )
 
 
@@ -105,6 +107,16 @@ def lib() -> ctypes.CDLL:
     L.lulesh_create_mixed.restype = I
     L.lulesh_create_scaled.argtypes = [ctypes.POINTER(Config), D, I, ctypes.POINTER(P)]
     L.lulesh_create_scaled.restype = I
+    L.lulesh_create_lmesh.argtypes = [ctypes.c_char_p, ctypes.POINTER(Config),
+                                      ctypes.POINTER(P)]
+    L.lulesh_create_lmesh.restype = I
+    L.lulesh_create_mesh.argtypes = ([I, I, DP, DP, DP, IP, IP] + [I, IP] * 4 +
+                                     [ctypes.POINTER(Config), ctypes.POINTER(P)])
+    L.lulesh_create_mesh.restype = I
+    L.lulesh_origin_element.argtypes = [P]
+    L.lulesh_origin_element.restype = I
+    L.lulesh_get_nodelist.argtypes = [P, IP]
+    L.lulesh_get_nodelist.restype = I
     L.lulesh_is_mixed.argtypes = [P]
     L.lulesh_is_mixed.restype = I
     L.lulesh_step.argtypes = [P, I, IP]
@@ -244,6 +256,54 @@ class Domain:
         st = self.stats()
         self.num_elem, self.num_node = st.num_elem, st.num_node
 
+    @classmethod
+    def from_lmesh(cls, path: str, stoptime: float = 1.0e-2,
+                   max_cycles: int = 9999999) -> "Domain":
+        """An unstructured mesh read from a .lmesh file (LULESH 1.0
+        semantics)."""
+        L = lib()
+        cfg = default_config()
+        cfg.ic_mode, cfg.stoptime, cfg.max_cycles = 1, stoptime, max_cycles
+        h = ctypes.c_void_p()
+        rc = L.lulesh_create_lmesh(path.encode(), ctypes.byref(cfg), ctypes.byref(h))
+        if rc != OK:
+            raise LuleshError(rc, f"lulesh_create_lmesh failed for {path}")
+        return cls._wrap(h, 1)
+
+    @classmethod
+    def from_mesh(cls, mesh: dict, stoptime: float = 1.0e-2,
+                  max_cycles: int = 9999999) -> "Domain":
+        """An unstructured mesh from arrays: x, y, z, nodelist [ne, 8],
+        neighbors [ne, 6], symm_x, symm_y, symm_z, free_surf."""
+        L = lib()
+        cfg = default_config()
+        cfg.ic_mode, cfg.stoptime, cfg.max_cycles = 1, stoptime, max_cycles
+        a = {k: np.ascontiguousarray(mesh[k], np.float64) for k in ("x", "y", "z")}
+        ints = {k: np.ascontiguousarray(mesh[k], np.int32)
+                for k in ("nodelist", "neighbors", "symm_x", "symm_y", "symm_z",
+                          "free_surf")}
+        ip = lambda v: v.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        h = ctypes.c_void_p()
+        rc = L.lulesh_create_mesh(
+            len(ints["nodelist"].reshape(-1, 8)), len(a["x"]), _dp(a["x"]), _dp(a["y"]),
+            _dp(a["z"]), ip(ints["nodelist"]), ip(ints["neighbors"]),
+            len(ints["symm_x"]), ip(ints["symm_x"]), len(ints["symm_y"]), ip(ints["symm_y"]),
+            len(ints["symm_z"]), ip(ints["symm_z"]), len(ints["free_surf"]),
+            ip(ints["free_surf"]), ctypes.byref(cfg), ctypes.byref(h))
+        if rc != OK:
+            raise LuleshError(rc, "lulesh_create_mesh failed")
+        return cls._wrap(h, 1)
+
+    @classmethod
+    def _wrap(cls, h, ic_mode: int) -> "Domain":
+        d = cls.__new__(cls)
+        d.h = h
+        d.nx, d.ic_mode, d.mixed = 0, ic_mode, False
+        st = d.stats()
+        d.num_elem, d.num_node = st.num_elem, st.num_node
+        d.origin = lib().lulesh_origin_element(h)
+        return d
+
     def close(self) -> None:
         if getattr(self, "h", None):
             lib().lulesh_destroy(self.h)
@@ -375,6 +435,13 @@ class Domain:
         if rc not in (OK, VOLUME_ERROR):
             self._check(rc)
         return f, q
+
+    def nodelist(self) -> np.ndarray:
+        """The element corners, [num_elem, 8] node indices."""
+        out = np.empty((self.stats().num_elem, 8), np.int32)
+        self._check(lib().lulesh_get_nodelist(
+            self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return out
 
     def write(self, path: str) -> None:
         """The state as a VTK file (lulesh_write_domain)."""

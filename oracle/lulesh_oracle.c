@@ -43,6 +43,8 @@ enum { LO_VOLUME_ERROR = -1, LO_QSTOP_ERROR = -2 };
 
 struct lo_domain {
     int nx, mode, num_elem, num_node, edge_nodes;
+    int num_symm_x, num_symm_y, num_symm_z; /* sizes of the symmetry node sets */
+    int origin;                             /* the element holding e(0) */
     int num_reg, balance, cost, cbrt_mode;
     double efactor; /* multiplies the initial origin energy (ensembles) */
 
@@ -532,7 +534,6 @@ int lo_lagrange_nodal(lo_domain *d)
     const double delt = d->deltatime;
     const double u_cut = d->u_cut;
     const int num_node = d->num_node;
-    const int nbc = d->edge_nodes * d->edge_nodes;
     int rc = calc_force_for_nodes(d);
     if (rc) {
         return rc;
@@ -543,13 +544,13 @@ int lo_lagrange_nodal(lo_domain *d)
         d->ydd[i] = d->fy[i] / d->nodal_mass[i];
         d->zdd[i] = d->fz[i] / d->nodal_mass[i];
     }
-    for (int i = 0; i < nbc; ++i) {
+    for (int i = 0; i < d->num_symm_x; ++i) {
         d->xdd[d->symm_x[i]] = 0.0;
     }
-    for (int i = 0; i < nbc; ++i) {
+    for (int i = 0; i < d->num_symm_y; ++i) {
         d->ydd[d->symm_y[i]] = 0.0;
     }
-    for (int i = 0; i < nbc; ++i) {
+    for (int i = 0; i < d->num_symm_z; ++i) {
         d->zdd[d->symm_z[i]] = 0.0;
     }
 #pragma omp parallel for
@@ -1252,6 +1253,8 @@ lo_domain *lo_create_scaled(int nx, int mode, int num_reg, int balance, int cost
     d->symm_x = ialloc((size_t)en * en);
     d->symm_y = ialloc((size_t)en * en);
     d->symm_z = ialloc((size_t)en * en);
+    d->num_symm_x = d->num_symm_y = d->num_symm_z = en * en;
+    d->origin = 0;
     d->nodelist = ialloc(8 * ne);
     d->lxim = ialloc(ne); d->lxip = ialloc(ne);
     d->letam = ialloc(ne); d->letap = ialloc(ne);
@@ -1446,6 +1449,178 @@ lo_domain *lo_create_scaled(int nx, int mode, int num_reg, int balance, int cost
     return d;
 }
 
+/* An unstructured hexahedral mesh, LULESH 1.0 semantics, set up as the
+ * reference's mesh reader sets up a .lmesh input
+ * (reference/CUDA/lulesh-kepler-singlegpu/lulesh.cu:728-925): coordinates,
+ * node lists, face neighbours and node sets from the caller; the element
+ * boundary masks derived from the node sets -- a face whose four nodes all
+ * lie in the union of the symmetry sets is a symmetry face, and so is (as
+ * the reader does it) a face whose four nodes all lie in the free-surface
+ * set; the energy deposited in the first element whose mask is exactly the
+ * three low symmetry faces (the octant corner). */
+lo_domain *lo_create_mesh(int num_elem, int num_node, const double *x, const double *y,
+                          const double *z, const int *nodelist, const int *neighbors,
+                          int nsx, const int *symm_x, int nsy, const int *symm_y,
+                          int nsz, const int *symm_z, int nfree, const int *free_surf)
+{
+    if (num_elem < 1 || num_node < 8 || !x || !y || !z || !nodelist || !neighbors ||
+        nsx < 0 || nsy < 0 || nsz < 0 || nfree < 0) return NULL;
+    for (size_t i = 0; i < 8 * (size_t)num_elem; ++i)
+        if (nodelist[i] < 0 || nodelist[i] >= num_node) return NULL;
+    for (size_t i = 0; i < 6 * (size_t)num_elem; ++i)
+        if (neighbors[i] < 0 || neighbors[i] >= num_elem) return NULL;
+    lo_domain *d = (lo_domain *)calloc(1, sizeof(lo_domain));
+    const size_t ne = (size_t)num_elem, nn = (size_t)num_node;
+    d->nx = 0;
+    d->mode = 1;
+    d->edge_nodes = 0;
+    d->num_elem = num_elem;
+    d->num_node = num_node;
+    d->num_reg = 1;
+    d->balance = 1;
+    d->cost = 1;
+    d->cbrt_mode = 0;
+    d->efactor = 1.0;
+    d->x = dalloc(nn); d->y = dalloc(nn); d->z = dalloc(nn);
+    d->xd = dalloc(nn); d->yd = dalloc(nn); d->zd = dalloc(nn);
+    d->xdd = dalloc(nn); d->ydd = dalloc(nn); d->zdd = dalloc(nn);
+    d->fx = dalloc(nn); d->fy = dalloc(nn); d->fz = dalloc(nn);
+    d->nodal_mass = dalloc(nn);
+    d->num_symm_x = nsx; d->num_symm_y = nsy; d->num_symm_z = nsz;
+    d->symm_x = ialloc((size_t)nsx + 1);
+    d->symm_y = ialloc((size_t)nsy + 1);
+    d->symm_z = ialloc((size_t)nsz + 1);
+    d->nodelist = ialloc(8 * ne);
+    d->lxim = ialloc(ne); d->lxip = ialloc(ne);
+    d->letam = ialloc(ne); d->letap = ialloc(ne);
+    d->lzetam = ialloc(ne); d->lzetap = ialloc(ne);
+    d->elem_bc = ialloc(ne);
+    d->e = dalloc(ne); d->p = dalloc(ne); d->q = dalloc(ne);
+    d->ql = dalloc(ne); d->qq = dalloc(ne); d->v = dalloc(ne);
+    d->volo = dalloc(ne); d->vnew = dalloc(ne); d->delv = dalloc(ne);
+    d->vdov = dalloc(ne); d->arealg = dalloc(ne); d->ss = dalloc(ne);
+    d->elem_mass = dalloc(ne);
+    d->dxx = dalloc(ne); d->dyy = dalloc(ne); d->dzz = dalloc(ne);
+    d->delv_xi = dalloc(ne); d->delv_eta = dalloc(ne); d->delv_zeta = dalloc(ne);
+    d->delx_xi = dalloc(ne); d->delx_eta = dalloc(ne); d->delx_zeta = dalloc(ne);
+    d->fx_elem = dalloc(8 * ne); d->fy_elem = dalloc(8 * ne); d->fz_elem = dalloc(8 * ne);
+    d->determ = dalloc(ne);
+    d->vnewc = dalloc(ne);
+
+    memcpy(d->x, x, nn * sizeof(double));
+    memcpy(d->y, y, nn * sizeof(double));
+    memcpy(d->z, z, nn * sizeof(double));
+    memcpy(d->nodelist, nodelist, 8 * ne * sizeof(int));
+    for (size_t i = 0; i < ne; ++i) {
+        d->lxim[i] = neighbors[6 * i + 0];
+        d->lxip[i] = neighbors[6 * i + 1];
+        d->letam[i] = neighbors[6 * i + 2];
+        d->letap[i] = neighbors[6 * i + 3];
+        d->lzetam[i] = neighbors[6 * i + 4];
+        d->lzetap[i] = neighbors[6 * i + 5];
+    }
+    if (nsx) memcpy(d->symm_x, symm_x, (size_t)nsx * sizeof(int));
+    if (nsy) memcpy(d->symm_y, symm_y, (size_t)nsy * sizeof(int));
+    if (nsz) memcpy(d->symm_z, symm_z, (size_t)nsz * sizeof(int));
+
+    /* node -> corner list in ascending element order */
+    {
+        int *count = ialloc(nn);
+        d->node_elem_start = ialloc(nn + 1);
+        for (size_t i = 0; i < ne; ++i)
+            for (int j = 0; j < 8; ++j) ++count[d->nodelist[8 * i + j]];
+        d->node_elem_start[0] = 0;
+        for (size_t i = 1; i <= nn; ++i)
+            d->node_elem_start[i] = d->node_elem_start[i - 1] + count[i - 1];
+        d->node_elem_corner = ialloc((size_t)d->node_elem_start[nn]);
+        memset(count, 0, nn * sizeof(int));
+        for (size_t i = 0; i < ne; ++i) {
+            for (int j = 0; j < 8; ++j) {
+                const int m = d->nodelist[8 * i + j];
+                d->node_elem_corner[d->node_elem_start[m] + count[m]] = (int)(i * 8 + j);
+                ++count[m];
+            }
+        }
+        free(count);
+    }
+    /* boundary masks from the node sets */
+    {
+        int *surf = ialloc(nn);
+        const int *sets[2][3] = {{symm_x, symm_y, symm_z}, {free_surf, NULL, NULL}};
+        const int sizes[2][3] = {{nsx, nsy, nsz}, {nfree, 0, 0}};
+        for (int pass = 0; pass < 2; ++pass) {
+            memset(surf, 0, nn * sizeof(int));
+            for (int s = 0; s < 3; ++s)
+                for (int i = 0; i < sizes[pass][s]; ++i) surf[sets[pass][s][i]] = 1;
+            for (size_t i = 0; i < ne; ++i) {
+                int mask = 0;
+                for (int j = 0; j < 8; ++j) mask |= surf[d->nodelist[8 * i + j]] << j;
+                if ((mask & 0x0f) == 0x0f) d->elem_bc[i] |= BC_ZETA_M_SYMM;
+                if ((mask & 0xf0) == 0xf0) d->elem_bc[i] |= BC_ZETA_P_SYMM;
+                if ((mask & 0x33) == 0x33) d->elem_bc[i] |= BC_ETA_M_SYMM;
+                if ((mask & 0xcc) == 0xcc) d->elem_bc[i] |= BC_ETA_P_SYMM;
+                if ((mask & 0x99) == 0x99) d->elem_bc[i] |= BC_XI_M_SYMM;
+                if ((mask & 0x66) == 0x66) d->elem_bc[i] |= BC_XI_P_SYMM;
+            }
+            if (pass == 0) {
+                d->origin = 0;
+                for (size_t i = 0; i < ne; ++i) {
+                    if (d->elem_bc[i] == (BC_XI_M_SYMM | BC_ETA_M_SYMM | BC_ZETA_M_SYMM)) {
+                        d->origin = (int)i;
+                        break;
+                    }
+                }
+            }
+        }
+        free(surf);
+    }
+
+    d->e_cut = 1.0e-7;
+    d->p_cut = 1.0e-7;
+    d->q_cut = 1.0e-7;
+    d->u_cut = 1.0e-7;
+    d->v_cut = 1.0e-10;
+    d->hgcoef = 3.0;
+    d->ss4o3 = 4.0 / 3.0;
+    d->qstop = 1.0e+12;
+    d->monoq_max_slope = 1.0;
+    d->monoq_limiter_mult = 2.0;
+    d->qlc_monoq = 0.5;
+    d->qqc_monoq = 2.0 / 3.0;
+    d->qqc = 2.0;
+    d->pmin = 0.;
+    d->emin = -1.0e+15;
+    d->dvovmax = 0.1;
+    d->eosvmax = 1.0e+9;
+    d->eosvmin = 1.0e-9;
+    d->refdens = 1.0;
+    d->dtfixed = -1.0e-7;
+    d->deltatimemultlb = 1.1;
+    d->deltatimemultub = 1.2;
+    d->stoptime = 1.0e-2;
+    d->dtcourant = 1.0e+20;
+    d->dthydro = 1.0e+20;
+    d->dtmax = 1.0e-2;
+    d->time = 0.;
+    d->cycle = 0;
+    for (size_t i = 0; i < ne; ++i) d->v[i] = 1.0;
+    for (size_t i = 0; i < ne; ++i) {
+        double xl[8], yl[8], zl[8];
+        const int *nl = &d->nodelist[8 * i];
+        gather8(d, d->x, nl, xl);
+        gather8(d, d->y, nl, yl);
+        gather8(d, d->z, nl, zl);
+        const double volume = elem_volume(xl, yl, zl);
+        d->volo[i] = volume;
+        d->elem_mass[i] = volume;
+        for (int j = 0; j < 8; ++j) d->nodal_mass[nl[j]] += volume / 8.0;
+    }
+    d->e[d->origin] = 3.948746e+7;
+    d->deltatime = 1.0e-7;
+    create_regions(d);
+    return d;
+}
+
 void lo_destroy(lo_domain *d)
 {
     if (!d) return;
@@ -1545,6 +1720,8 @@ double lo_scalar(const lo_domain *d, const char *name)
     if (strcmp(name, "dthydro") == 0) return d->dthydro;
     if (strcmp(name, "stoptime") == 0) return d->stoptime;
     if (strcmp(name, "cycle") == 0) return (double)d->cycle;
+    if (strcmp(name, "origin") == 0) return (double)d->origin;
+    if (strcmp(name, "e_origin") == 0) return d->e[d->origin];
     return NAN;
 }
 

@@ -33,6 +33,16 @@ lo_domain *lo_create(int nx, int mode, int num_reg, int balance, int cost);
  * as the last factor of the energy product. */
 lo_domain *lo_create_scaled(int nx, int mode, int num_reg, int balance, int cost,
                             double efactor);
+/* An unstructured hexahedral mesh in mode 1 (LULESH 1.0), as the
+ * reference's .lmesh reader builds it (reference/CUDA/
+ * lulesh-kepler-singlegpu/lulesh.cu:728-925): nodelist [num_elem][8],
+ * neighbors [num_elem][6] (xi-, xi+, eta-, eta+, zeta-, zeta+), the
+ * symmetry node sets and the free-surface node set.  lo_scalar(d, "origin")
+ * is the element holding the initial energy. */
+lo_domain *lo_create_mesh(int num_elem, int num_node, const double *x, const double *y,
+                          const double *z, const int *nodelist, const int *neighbors,
+                          int nsx, const int *symm_x, int nsy, const int *symm_y,
+                          int nsz, const int *symm_z, int nfree, const int *free_surf);
 void lo_destroy(lo_domain *d);
 
 /* cube root used by the hourglass coefficient: 0 = C library cbrt (the

@@ -50,6 +50,10 @@ def lib() -> ctypes.CDLL:
         L.lo_create.argtypes = [ctypes.c_int] * 5
         L.lo_create_scaled.restype = P
         L.lo_create_scaled.argtypes = [ctypes.c_int] * 5 + [ctypes.c_double]
+        IPT = ctypes.POINTER(ctypes.c_int)
+        L.lo_create_mesh.restype = P
+        L.lo_create_mesh.argtypes = ([ctypes.c_int, ctypes.c_int, D, D, D, IPT, IPT] +
+                                     [ctypes.c_int, IPT] * 4)
         L.lo_destroy.argtypes = [P]
         L.lo_set_cbrt_mode.argtypes = [P, ctypes.c_int]
         L.lo_set_stoptime.argtypes = [P, ctypes.c_double]
@@ -97,8 +101,90 @@ def iptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
 
 
+def parse_lmesh(text: str) -> dict:
+    """The lists of a .lmesh file (the format read by reference/CUDA/
+    lulesh-kepler-singlegpu/lulesh.cu:728-848)."""
+    tok = text.split()
+    pos = 0
+
+    def take(n, conv):
+        nonlocal pos
+        out = [conv(t) for t in tok[pos:pos + n]]
+        if len(out) != n:
+            raise ValueError("truncated .lmesh")
+        pos += n
+        return out
+
+    ne, nn = take(2, int)
+    xyz = np.array(take(3 * nn, float)).reshape(nn, 3)
+    mesh = {"x": xyz[:, 0].copy(), "y": xyz[:, 1].copy(), "z": xyz[:, 2].copy(),
+            "nodelist": np.array(take(8 * ne, int), np.int32).reshape(ne, 8),
+            "neighbors": np.array(take(6 * ne, int), np.int32).reshape(ne, 6)}
+    for name in ("symm_x", "symm_y", "symm_z", "free_surf"):
+        (n,) = take(1, int)
+        mesh[name] = np.array(take(n, int), np.int32)
+    return mesh
+
+
+def cube_mesh(nx: int) -> dict:
+    """The Sedov cube nx^3 written as explicit lists (the structured
+    connectivity and node sets of reference/C/LULESH-OMP.cc:2996-3151); the
+    far faces' nodes form the free-surface set."""
+    en = nx + 1
+    k, j, i = np.meshgrid(np.arange(en), np.arange(en), np.arange(en), indexing="ij")
+    coord = np.array([0.0] + [1.125 * c / nx for c in range(1, en)])
+    x, y, z = coord[i].ravel(), coord[j].ravel(), coord[k].ravel()
+    ek, ej, ei = [a.ravel() for a in np.meshgrid(np.arange(nx), np.arange(nx),
+                                                  np.arange(nx), indexing="ij")]
+    n0 = (ek * en + ej) * en + ei
+    pl = en * en
+    nodelist = np.stack([n0, n0 + 1, n0 + en + 1, n0 + en, n0 + pl, n0 + pl + 1,
+                         n0 + pl + en + 1, n0 + pl + en], axis=1).astype(np.int32)
+    el = np.arange(nx ** 3)
+    nb = np.stack([np.where(ei > 0, el - 1, el), np.where(ei < nx - 1, el + 1, el),
+                   np.where(ej > 0, el - nx, el), np.where(ej < nx - 1, el + nx, el),
+                   np.where(ek > 0, el - nx * nx, el),
+                   np.where(ek < nx - 1, el + nx * nx, el)], axis=1).astype(np.int32)
+    nid = np.arange(en ** 3)
+    far = (i.ravel() == nx) | (j.ravel() == nx) | (k.ravel() == nx)
+    return {"x": x, "y": y, "z": z, "nodelist": nodelist, "neighbors": nb,
+            "symm_x": nid[i.ravel() == 0].astype(np.int32),
+            "symm_y": nid[j.ravel() == 0].astype(np.int32),
+            "symm_z": nid[k.ravel() == 0].astype(np.int32),
+            "free_surf": nid[far].astype(np.int32)}
+
+
 class Oracle:
-    """One oracle domain (a Sedov cube of nx^3 elements)."""
+    """One oracle domain (a Sedov cube of nx^3 elements, or an unstructured
+    mesh through from_mesh)."""
+
+    @classmethod
+    def from_mesh(cls, mesh: dict, cbrt_mode: int = 0, stoptime: float | None = None,
+                  threads: int = 0) -> "Oracle":
+        L = lib()
+        L.lo_set_threads(threads)
+        a = {k: np.ascontiguousarray(mesh[k], np.float64) for k in ("x", "y", "z")}
+        ints = {k: np.ascontiguousarray(mesh[k], np.int32)
+                for k in ("nodelist", "neighbors", "symm_x", "symm_y", "symm_z",
+                          "free_surf")}
+        ne = len(ints["nodelist"].reshape(-1, 8))
+        h = L.lo_create_mesh(ne, len(a["x"]), dptr(a["x"]), dptr(a["y"]), dptr(a["z"]),
+                             iptr(ints["nodelist"]), iptr(ints["neighbors"]),
+                             len(ints["symm_x"]), iptr(ints["symm_x"]),
+                             len(ints["symm_y"]), iptr(ints["symm_y"]),
+                             len(ints["symm_z"]), iptr(ints["symm_z"]),
+                             len(ints["free_surf"]), iptr(ints["free_surf"]))
+        if not h:
+            raise ValueError("lo_create_mesh failed")
+        o = cls.__new__(cls)
+        o.h = h
+        o.nx, o.mode = 0, 1
+        L.lo_set_cbrt_mode(h, cbrt_mode)
+        if stoptime is not None:
+            L.lo_set_stoptime(h, stoptime)
+        o.num_elem = L.lo_num_elem(h)
+        o.num_node = L.lo_num_node(h)
+        return o
 
     def __init__(self, nx: int, mode: int = 2, num_reg: int = 11,
                  balance: int = 1, cost: int = 1, cbrt_mode: int = 0,

@@ -40,3 +40,17 @@ def test_harness_1_0_and_cycle_limit():
 def test_harness_rejects_bad_options():
     out = subprocess.run([HARNESS, "-s", "abc"], capture_output=True, text=True)
     assert out.returncode != 0
+
+
+def test_harness_unstructured_mesh(tmp_path):
+    import lulesh_oracle
+    mesh = os.path.join(ROOT, "tests", "golden", "sedov15oct.lmesh")
+    vtk = str(tmp_path / "end.vtk")
+    res = lulesh_oracle.parse_reference_output(run(["-u", mesh, "--vtk", vtk]))
+    o = lulesh_oracle.Oracle.from_mesh(lulesh_oracle.parse_lmesh(open(mesh).read()),
+                                       cbrt_mode=1, threads=8)
+    o.run()
+    assert res["cycles"] == o.cycle
+    e0 = lulesh_oracle.lib().lo_scalar(o.h, b"e_origin")
+    assert f"{res['origin_energy']:12.6e}" == f"{e0:12.6e}"
+    assert os.path.getsize(vtk) > 0
