@@ -18,10 +18,11 @@ workload.
 The "forms" object holds the other forms of the step on their own
 workloads: the ensemble (64 Sedov members of uneven size, each to t=1e-2)
 the mixed-precision step (120^3 to t=1e-2, with its errors against the
-float64 run) and the learned-Q closure (40^3 against the 120^3 solution).
+float64 run), the learned-Q closure (40^3 against the 120^3 solution) and
+the unstructured step (the 150^3 cube given as explicit mesh lists).
 
     python bench.py [--steps K] [--warmup W] [--nx N] [--no-cpu-baseline]
-                    [--no-forms]
+                    [--no-forms | --forms batched,mixed,learned_q,unstructured]
 """
 from __future__ import annotations
 
@@ -166,6 +167,102 @@ def bench_batched(lulesh_amd, plain_value: float) -> dict:
     }
 
 
+# The unstructured step's kernels (lulesh_create_mesh): corner forces
+# (nodelist 32, p,q,v,volo,ss,elemMass 48, write stress and hourglass
+# corner forces 384 per element; x..zd 48 per node), the node gather
+# (corner list 32 + the corner forces 384 per element; corner range 4,
+# mass 8, symmetry byte 1, read xd,x 48 and write them 48 per node), the
+# kinematics with the nodelist (176 + 32) and Q/EOS/dt with the face
+# neighbours, face masks and EOS schedule instead of the region index
+# (264 - 12 + 24 + 4 + 4 + 2).  Sum: 1374 B/element + 253 B/node.
+KERNEL_BYTES_U = {
+    "k_corner_forces_u": (464, 48),
+    "k_nodes_u": (416, 109),
+    "k_kinematics_q": (208, 96),
+    "k_q_eos_dt": (286, 0),
+    "k_time_increment": (0, 0),
+}
+
+
+def cube_lists(nx: int) -> dict:
+    """The Sedov cube nx^3 as explicit mesh lists (node coordinates,
+    nodelist, face neighbours, symmetry and free-surface node sets), the
+    form a .lmesh file holds."""
+    import numpy as np
+    en = nx + 1
+    coord = np.array([0.0] + [1.125 * c / nx for c in range(1, en)])
+    k, j, i = (a.ravel() for a in np.meshgrid(np.arange(en), np.arange(en), np.arange(en),
+                                                indexing="ij"))
+    ek, ej, ei = (a.ravel() for a in np.meshgrid(np.arange(nx), np.arange(nx), np.arange(nx),
+                                                   indexing="ij"))
+    n0 = (ek * en + ej) * en + ei
+    pl = en * en
+    nodelist = np.stack([n0, n0 + 1, n0 + en + 1, n0 + en, n0 + pl, n0 + pl + 1,
+                         n0 + pl + en + 1, n0 + pl + en], axis=1).astype(np.int32)
+    el = np.arange(nx ** 3)
+    nb = np.stack([np.where(ei > 0, el - 1, el), np.where(ei < nx - 1, el + 1, el),
+                   np.where(ej > 0, el - nx, el), np.where(ej < nx - 1, el + nx, el),
+                   np.where(ek > 0, el - nx * nx, el),
+                   np.where(ek < nx - 1, el + nx * nx, el)], axis=1).astype(np.int32)
+    nid = np.arange(en ** 3, dtype=np.int32)
+    return {"x": coord[i], "y": coord[j], "z": coord[k], "nodelist": nodelist,
+            "neighbors": nb, "symm_x": nid[i == 0], "symm_y": nid[j == 0],
+            "symm_z": nid[k == 0], "free_surf": nid[(i == nx) | (j == nx) | (k == nx)]}
+
+
+def bench_unstructured(lulesh_amd, nx: int = 150, steps: int = 50, warmup: int = 5) -> dict:
+    """The unstructured step (explicit connectivity, corner forces gathered
+    per node) on the Sedov cube nx^3 given as mesh lists, LULESH 1.0 ICs,
+    beside the structured step on the same cube: zones*cycles/s over
+    `steps` cycles and the roofline of each unstructured kernel."""
+    ne, nn = nx ** 3, (nx + 1) ** 3
+    kt = 20
+    out = {}
+    lists = cube_lists(nx)
+    for name in ("unstructured", "structured"):
+        if name == "unstructured":
+            d = lulesh_amd.Domain.from_mesh(lists, max_cycles=warmup + steps + kt)
+        else:
+            d = lulesh_amd.Domain(nx, ic_mode=1, max_cycles=warmup + steps + kt)
+        d.step(warmup)
+        t0 = time.perf_counter()
+        rc, n = d.step(steps)
+        wall = time.perf_counter() - t0
+        if rc != 0 or n != steps:
+            raise RuntimeError(f"{name} step returned rc={rc}, {n} of {steps} cycles")
+        out[name] = {"value": ne * steps / wall, "ms_per_cycle": wall * 1e3 / steps}
+        if name == "unstructured":
+            d.set_kernel_timing(True)
+            d.step(kt)
+            times = d.kernel_times()
+            kern = {}
+            for k, (ms, cnt) in times.items():
+                if not cnt or k not in KERNEL_BYTES_U:
+                    continue
+                be, bn = KERNEL_BYTES_U[k]
+                mean = ms / cnt
+                b = be * ne + bn * nn
+                kern[k] = {"mean_ms": mean, "alg_bytes": b,
+                           "achieved_gbs": b / (mean * 1e-3) / 1e9 if b else 0.0,
+                           "frac": b / (mean * 1e-3) / 1e9 / HBM_PEAK_GBS if b else 0.0}
+            out[name]["kernels"] = kern
+        d.close()
+        print(f"bench: {name} {nx}^3: {out[name]['ms_per_cycle']:.3f} ms/cycle",
+              file=sys.stderr, flush=True)
+    b_cycle = 1374 * ne + 253 * nn
+    roof = HBM_PEAK_GBS * 1e9 / b_cycle * ne
+    u = out["unstructured"]
+    return {
+        "value": u["value"], "unit": "zones·cycles/s",
+        "workload": f"sedov_{nx}^3 as explicit mesh lists, LULESH 1.0 ICs, cycles "
+                    f"{warmup + 1}-{warmup + steps}",
+        "unstructured": u, "structured_same_cube": out["structured"],
+        "vs_structured": u["value"] / out["structured"]["value"],
+        "cycle_roofline": {"b_alg_per_cycle": b_cycle, "roofline_zones_cycles_per_s": roof,
+                           "frac": u["value"] / roof},
+    }
+
+
 def bench_mixed(lulesh_amd, nx: int = 120) -> dict:
     """The mixed-precision form on its workload: Sedov at nx^3 (LULESH 2.0
     ICs, 11 regions), run to t = 1e-2 in float64 and in mixed precision;
@@ -239,7 +336,9 @@ def main() -> int:
     ap.add_argument("--cpu-cycles", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true",
-                    help="skip the ensemble / mixed / learned-Q forms")
+                    help="skip the other forms of the step")
+    ap.add_argument("--forms", default="batched,mixed,learned_q,unstructured",
+                    help="comma-separated forms to run after the plain step")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not "
@@ -319,12 +418,23 @@ def main() -> int:
         "gpu_event_ms_per_step": sum(ms for ms, _ in times.values()) / KT,
         "breakdown_cycles": KT,
     }
-    if not args.no_forms:
-        result["forms"] = {"batched": bench_batched(lulesh_amd, value),
-                           "mixed": bench_mixed(lulesh_amd)}
+    forms = [] if args.no_forms else [f for f in args.forms.split(",") if f]
+    unknown = set(forms) - {"batched", "mixed", "learned_q", "unstructured"}
+    if unknown:
+        print(f"bench: unknown forms {sorted(unknown)}", file=sys.stderr)
+        return 2
+    if forms:
+        result["forms"] = {}
+    if "batched" in forms:
+        result["forms"]["batched"] = bench_batched(lulesh_amd, value)
+    if "mixed" in forms:
+        result["forms"]["mixed"] = bench_mixed(lulesh_amd)
         result["forms"]["mixed"]["vs_plain_150"] = (
             result["forms"]["mixed"]["value"] / value)
+    if "learned_q" in forms:
         result["forms"]["learned_q"] = bench_learned_q(lulesh_amd)
+    if "unstructured" in forms:
+        result["forms"]["unstructured"] = bench_unstructured(lulesh_amd)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(nx, args.cpu_cycles)
     print(json.dumps(result))
