@@ -48,6 +48,13 @@ constexpr SweepShape kDefaultSweep = {32, 8, 16, 2};
 // on a CU
 constexpr SweepShape kDefaultSweepMixed = {32, 8, 12, 3};
 
+// consecutive Q/EOS blocks take chunks from this many bands of the
+// region-sorted schedule (odd: blocks go to the 8 XCDs round-robin, and an
+// even count would send the costly band to only some of them; 3 and 5 are
+// 8% faster than 1 at 150^3, 2, 4 and 8 up to 2.5x slower);
+// lulesh_set_eos_bands overrides it
+constexpr int kDefaultEosBands = 3;
+
 constexpr int kBlock = 256;
 constexpr int kBatch = 32;  // cycles queued per host round trip in run()
 
@@ -70,6 +77,7 @@ struct lulesh_domain {
     DevMeshU mesh{};
     int origin = 0;             // the element holding the initial energy
     std::vector<void *> bufs;   // device allocations of an unstructured domain
+    int eos_bands = kDefaultEosBands;   // EOS schedule interleave (eos_chunk)
     const MlpFrags *learned = nullptr;  // learned-Q network, if set
     float learned_z_max = 0.f;
     DevFields f{};
@@ -215,12 +223,13 @@ int queue_cycle(lulesh_domain *d)
     if ((rc = launch(d, KID_Q_EOS_DT, [&] {
              if (mixed) {
                  hipLaunchKernelGGL(k_q_eos_dt_mixed, dim3(ge), dim3(kBlock), 0, st, fm, s,
-                                    p);
+                                    p, d->eos_bands);
              } else if (d->learned) {
                  hipLaunchKernelGGL(k_q_eos_dt_learned, dim3(ge), dim3(kBlock), 0, st, f, s,
                                     p, d->learned, d->learned_z_max);
              } else {
-                 hipLaunchKernelGGL(k_q_eos_dt, dim3(ge), dim3(kBlock), 0, st, f, s, p);
+                 hipLaunchKernelGGL(k_q_eos_dt, dim3(ge), dim3(kBlock), 0, st, f, s, p,
+                                    d->eos_bands);
              }
          })))
         return rc;
@@ -888,6 +897,15 @@ int lulesh_set_parameter(lulesh_domain *d, const char *name, double value)
         return fail(d, LULESH_BAD_ARG, std::string("unknown parameter ") + name);
     }
     return write_scalars(d, s);
+}
+
+int lulesh_set_eos_bands(lulesh_domain *d, int bands)
+{
+    if (!d || bands < 1 || bands > 4096) {
+        return LULESH_BAD_ARG;
+    }
+    d->eos_bands = bands;
+    return LULESH_OK;
 }
 
 int lulesh_set_sweep_shape(lulesh_domain *d, int tx, int ty, int nz, int waves)

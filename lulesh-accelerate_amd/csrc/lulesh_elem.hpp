@@ -261,20 +261,22 @@ __device__ __forceinline__ T characteristic_length(const T *x, const T *y, const
 struct EosConst {
     double e_cut, p_cut, q_cut, eosvmax, eosvmin, pmin, emin, rho0;
     double ss_tiny, ss_floor;
-    bool unit_rho0;  // rho0 == 1: x / rho0 is x exactly, the divide is skipped
+    bool unit_rho0;  // rho0 == 1: x / rho0 is x exactly (see eos_element)
 };
 
 struct EosOut {
     double p, e, q, ss;
 };
 
+// the sound speed squared over rho0; UnitRho0 (rho0 == 1, where x / rho0
+// is x exactly) leaves the divide out at compile time
+template <bool UnitRho0>
 __device__ __forceinline__ double eos_sound(double pbvc, double e, double vv,
                                             double bvc, double p, double rho0,
-                                            bool unit_rho0, double tiny,
-                                            double floor_v)
+                                            double tiny, double floor_v)
 {
     double s = (pbvc * e + vv * bvc * p);
-    if (!unit_rho0) {
+    if (!UnitRho0) {
         s = s / rho0;
     }
     if (s <= tiny) {
@@ -295,6 +297,10 @@ __device__ __forceinline__ double eos_pressure(double bvc, double e,
     return p;
 }
 
+// UnitRho0 must equal c.unit_rho0: the caller picks the instance, so the
+// unit-density case carries no divides by rho0 and no selects between the
+// two results
+template <bool UnitRho0>
 __device__ __forceinline__ EosOut eos_element(double vnewc, double e_old,
                                               double delvc, double p_old,
                                               double q_old, double qq_old,
@@ -327,8 +333,8 @@ __device__ __forceinline__ EosOut eos_element(double vnewc, double e_old,
     if (delvc > 0.) {
         q_new = 0.;
     } else {
-        const double ssc = eos_sound(pbvc, e_new, vhalf * vhalf, bvc, p_half_step,
-                                     c.rho0, c.unit_rho0, c.ss_tiny, c.ss_floor);
+        const double ssc = eos_sound<UnitRho0>(pbvc, e_new, vhalf * vhalf, bvc, p_half_step,
+                                               c.rho0, c.ss_tiny, c.ss_floor);
         q_new = (ssc * ql_old + qq_old);
     }
     e_new = e_new + 0.5 * delvc * (3.0 * (p_old + q_old) - 4.0 * (p_half_step + q_new));
@@ -345,8 +351,8 @@ __device__ __forceinline__ EosOut eos_element(double vnewc, double e_old,
         if (delvc > 0.) {
             q_tilde = 0.;
         } else {
-            const double ssc = eos_sound(pbvc, e_new, vnewc * vnewc, bvc, p_new,
-                                         c.rho0, c.unit_rho0, c.ss_tiny, c.ss_floor);
+            const double ssc = eos_sound<UnitRho0>(pbvc, e_new, vnewc * vnewc, bvc, p_new,
+                                                   c.rho0, c.ss_tiny, c.ss_floor);
             q_tilde = (ssc * ql_old + qq_old);
         }
         e_new = e_new - (7.0 * (p_old + q_old) - 8.0 * (p_half_step + q_new) +
@@ -358,14 +364,14 @@ __device__ __forceinline__ EosOut eos_element(double vnewc, double e_old,
     pbvc = c1s;
     p_new = eos_pressure(bvc, e_new, vnewc, c);
     if (delvc <= 0.) {
-        const double ssc = eos_sound(pbvc, e_new, vnewc * vnewc, bvc, p_new,
-                                     c.rho0, c.unit_rho0, c.ss_tiny, c.ss_floor);
+        const double ssc = eos_sound<UnitRho0>(pbvc, e_new, vnewc * vnewc, bvc, p_new,
+                                               c.rho0, c.ss_tiny, c.ss_floor);
         q_new = (ssc * ql_old + qq_old);
         if (fabs(q_new) < c.q_cut) q_new = 0.;
     }
     EosOut o;
-    o.ss = eos_sound(pbvc, e_new, vnewc * vnewc, bvc, p_new, c.rho0, c.unit_rho0,
-                     c.ss_tiny, c.ss_floor);
+    o.ss = eos_sound<UnitRho0>(pbvc, e_new, vnewc * vnewc, bvc, p_new, c.rho0, c.ss_tiny,
+                               c.ss_floor);
     o.p = p_new;
     o.e = e_new;
     o.q = q_new;

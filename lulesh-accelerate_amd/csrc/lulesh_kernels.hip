@@ -18,6 +18,8 @@
 // Every kernel returns at once when the cycle is not active (the run is
 // over or an earlier phase raised an error), so whole batches of cycles can
 // be queued without a host round trip per cycle.
+#include <type_traits>
+
 #include "lulesh_kernels.hpp"
 
 #include "lulesh_elem.hpp"
@@ -530,18 +532,27 @@ __device__ __forceinline__ void eos_dt_elem(const DevFields &f, DevScalars *s,
         EosOut o;
         o.p = o.e = o.q = o.ss = 0.0;
         // the region's repetitions redo the same evaluation, as the
-        // reference does (lulesh-2.0.3/lulesh.cc:2267).  The empty asm
-        // statements hide that the inputs are unchanged and mark every
-        // repetition's result as used, so each one is computed; two
-        // repetitions at a time give the scheduler independent work.
-#pragma unroll 2
-        for (int r = 0; r < rep; ++r) {
+        // reference does (lulesh-2.0.3/lulesh.cc:2267).  The inputs pass
+        // through an empty asm statement at the top of every repetition, so
+        // the compiler cannot know they are unchanged and computes each
+        // repetition in full; they are carried from one repetition to the
+        // next in the same registers (no copies), and every repetition's
+        // result is marked as used.
+        auto repeat = [&](auto unit) {
+            constexpr bool kUnit = decltype(unit)::value;
             double a0 = vnewc, a1 = e_old, a2 = delvc, a3 = p_old, a4 = q_old;
             double a5 = qquad, a6 = qlin;
-            asm volatile("" : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(a4),
-                         "+v"(a5), "+v"(a6));
-            o = eos_element(a0, a1, a2, a3, a4, a5, a6, ec);
-            asm volatile("" ::"v"(o.p), "v"(o.e), "v"(o.q), "v"(o.ss));
+            for (int r = 0; r < rep; ++r) {
+                asm volatile("" : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(a4),
+                             "+v"(a5), "+v"(a6));
+                o = eos_element<kUnit>(a0, a1, a2, a3, a4, a5, a6, ec);
+                asm volatile("" ::"v"(o.p), "v"(o.e), "v"(o.q), "v"(o.ss));
+            }
+        };
+        if (ec.unit_rho0) {
+            repeat(std::true_type{});
+        } else {
+            repeat(std::false_type{});
         }
         f.p[el] = o.p;
         f.e[el] = o.e;
@@ -615,7 +626,7 @@ __device__ __forceinline__ void q_eos_dt_block(const DevFields &f, DevScalars *s
 }  // namespace
 
 __global__ __launch_bounds__(256) void k_q_eos_dt(DevFields f, DevScalars *s,
-                                                  PhysParams prm)
+                                                  PhysParams prm, int bands)
 {
     __shared__ unsigned long long lds_c[4], lds_h[4];
     __shared__ int skip;
@@ -629,9 +640,7 @@ __global__ __launch_bounds__(256) void k_q_eos_dt(DevFields f, DevScalars *s,
     if (skip) {
         return;
     }
-    // the schedule's costly regions come last; blocks take it from the end
-    // so the long blocks start first and the short ones fill the tail
-    const int chunk = int(gridDim.x) - 1 - int(blockIdx.x);
+    const int chunk = eos_chunk(int(blockIdx.x), int(gridDim.x), bands);
     q_eos_dt_block(f, s, prm, chunk * int(blockDim.x) + int(threadIdx.x), lds_c, lds_h);
 }
 

@@ -76,6 +76,28 @@ __device__ __forceinline__ unsigned int fast_div(unsigned int n, const FastDiv &
     return (__umulhi(n, f.m) + n) >> f.l;
 }
 
+// The chunk of the EOS schedule that block b of g takes.  The schedule holds
+// the elements sorted by region, the costly regions (more EOS repetitions)
+// last; blocks take it from the end, so long blocks start first.  With
+// bands > 1 the reversed schedule is cut into `bands` near-equal bands
+// (the first g % bands one chunk longer) and consecutive blocks take the
+// next chunk of each band in turn, so compute-bound chunks of the costly
+// regions run beside memory-bound chunks of the cheap ones.  A bijection of
+// [0, g) for every bands >= 1.
+__device__ __forceinline__ int eos_chunk(int b, int g, int bands)
+{
+    const int q = g / bands, r = g - q * bands;
+    int band, pos;
+    if (b < q * bands) {
+        band = b % bands;
+        pos = b / bands;
+    } else {
+        band = b - q * bands;
+        pos = q;
+    }
+    return g - 1 - (band * q + min(band, r) + pos);
+}
+
 struct DevFields {
     int nx;
     int num_elem, num_node;
@@ -139,7 +161,7 @@ struct BatchWork {
 
 __global__ void k_time_increment(DevScalars *s, int ic_mode);
 __global__ void k_kinematics_q(DevFields f, DevScalars *s, PhysParams prm);
-__global__ void k_q_eos_dt(DevFields f, DevScalars *s, PhysParams prm);
+__global__ void k_q_eos_dt(DevFields f, DevScalars *s, PhysParams prm, int bands);
 __global__ void k_dt_finish(DevScalars *s, int ic_mode);
 __global__ void k_device_math(int op, int n, const double *a, const double *b,
                               double *out);
@@ -198,7 +220,7 @@ __global__ void k_q_eos_dt_learned(DevFields f, DevScalars *s, PhysParams prm,
 
 // The mixed-precision phases (the time step kernels are shared).
 __global__ void k_kinematics_q_mixed(DevFieldsMixed f, DevScalars *s, PhysParams prm);
-__global__ void k_q_eos_dt_mixed(DevFieldsMixed f, DevScalars *s, PhysParams prm);
+__global__ void k_q_eos_dt_mixed(DevFieldsMixed f, DevScalars *s, PhysParams prm, int bands);
 hipError_t launch_force_sweep_mixed(const SweepShape &shape, const DevFieldsMixed &f,
                                     DevScalars *s, const PhysParams &prm,
                                     hipStream_t stream);

@@ -666,7 +666,7 @@ __global__ __launch_bounds__(256) void k_kinematics_q_mixed(DevFieldsMixed f,
 // k_q_eos_dt (lulesh_kernels.hip)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_q_eos_dt_mixed(DevFieldsMixed f, DevScalars *s,
-                                                        PhysParams prm)
+                                                        PhysParams prm, int bands)
 {
     __shared__ unsigned long long lds_c[4], lds_h[4];
     __shared__ int skip;
@@ -677,7 +677,7 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_mixed(DevFieldsMixed f, DevSca
     if (skip) {
         return;
     }
-    const int slot = (int(gridDim.x) - 1 - int(blockIdx.x)) * int(blockDim.x) +
+    const int slot = eos_chunk(int(blockIdx.x), int(gridDim.x), bands) * int(blockDim.x) +
                      int(threadIdx.x);
     unsigned long long cand_c = as_bits(kBig), cand_h = as_bits(kBig);
     if (slot < f.num_elem) {

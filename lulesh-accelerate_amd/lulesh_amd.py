@@ -80,7 +80,7 @@ EXPORTS = (
     "lulesh_mlp_train", "lulesh_mlp_last_error", "lulesh_set_learned_q",
     "lulesh_q_features", "lulesh_write_domain_arrays", "lulesh_write_domain",
     "lulesh_create_lmesh", "lulesh_create_mesh", "lulesh_origin_element",
-    "lulesh_get_nodelist",
+    "lulesh_get_nodelist", "lulesh_set_eos_bands",
 )
 
 
@@ -117,6 +117,8 @@ def lib() -> ctypes.CDLL:
     L.lulesh_origin_element.restype = I
     L.lulesh_get_nodelist.argtypes = [P, IP]
     L.lulesh_get_nodelist.restype = I
+    L.lulesh_set_eos_bands.argtypes = [P, I]
+    L.lulesh_set_eos_bands.restype = I
     L.lulesh_is_mixed.argtypes = [P]
     L.lulesh_is_mixed.restype = I
     L.lulesh_step.argtypes = [P, I, IP]
@@ -381,6 +383,10 @@ class Domain:
 
     def set_sweep_shape(self, tx: int, ty: int, nz: int, waves: int) -> None:
         self._check(lib().lulesh_set_sweep_shape(self.h, tx, ty, nz, waves))
+
+    def set_eos_bands(self, bands: int) -> None:
+        """Interleave of the Q/EOS schedule (lulesh_set_eos_bands)."""
+        self._check(lib().lulesh_set_eos_bands(self.h, int(bands)))
 
     def sweep_shape(self) -> tuple[int, int, int, int]:
         v = [ctypes.c_int() for _ in range(4)]
