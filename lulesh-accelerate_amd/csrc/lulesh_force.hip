@@ -145,8 +145,15 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
 
     // fold the three staged components into the owned node:
     // F = (sum carried from the layer below) + corners 2,3,1,0; the carried
-    // sum for the plane above restarts from corners 6,7,5,4
-    auto fold = [&](bool finalize, bool carry, bool layer, double *F, double *P) {
+    // sum for the plane above restarts from corners 6,7,5,4.  The last fold
+    // of a layer also moves the node planes on (`advance`): the owned node of
+    // plane k is copied to registers before its barrier, since plane k's
+    // slot takes plane k+2 after it.
+    auto fold = [&](bool finalize, bool carry, bool layer, double *F, double *P, bool last,
+                    auto &&advance) {
+        if (last) {
+            advance(0);
+        }
         __syncthreads();
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
@@ -169,6 +176,9 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
                 P[t] = acc;
             }
         }
+        if (last) {
+            advance(1);
+        }
         __syncthreads();
     };
 
@@ -176,28 +186,80 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
     const int nodeoff[4] = {base, base + 1, base + PX + 1, base + PX};
 
     double P[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    // the element fields of a layer (p, q, v, volo, ss, elemMass) and the
+    // owned node's mass of a plane are loaded one iteration ahead too: the
+    // barriers keep the compiler from moving loads up, so a load issued in
+    // the iteration that uses it would wait for HBM
+    double epre[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double mpre = 1.0;
+    auto fetch_elem = [&](int kk) {
+        if (kk < nx && col_valid) {
+            const int e = (kk * nx + je) * nx + ie;
+            epre[0] = f.p[e];
+            epre[1] = f.q[e];
+            epre[2] = f.v[e];
+            epre[3] = f.volo[e];
+            epre[4] = f.ss[e];
+            epre[5] = f.elem_mass[e];
+        }
+    };
+    auto fetch_mass = [&](int kk) {
+        if (owner && kk >= kc0 && kk < kc1) {
+            mpre = f.nodal_mass[(kk * en + je) * en + ie];
+        }
+    };
+    // iteration k uses node planes k (slot k & 1) and, for an element layer
+    // (k < nx), k+1 (the other slot); both are in LDS when it starts, and
+    // plane k+2 waits in registers
     const int kbeg = (kc0 > 0) ? kc0 - 1 : 0;
+    fetch_elem(kbeg);
+    fetch_mass(kbeg);
     fetch_plane(kbeg);
     store_plane(kbeg & 1);
     if (kbeg < nx) {
         fetch_plane(kbeg + 1);
+        store_plane((kbeg + 1) & 1);
+        if (kbeg + 1 < nx && kbeg + 1 < kc1) {
+            fetch_plane(kbeg + 2);
+        }
     }
+    __syncthreads();
     for (int k = kbeg; k < kc1; ++k) {
         const bool layer = k < nx;
-        __syncthreads();  // the readers of the slot below are done
-        if (layer) {
-            store_plane((k + 1) & 1);  // plane k+1, fetched one layer ago
-        }
-        __syncthreads();
-        if (layer && k + 1 < nx && k + 1 < kc1) {
-            fetch_plane(k + 2);  // in flight while this layer computes
-        }
         const int lo = k & 1;
         const int hi = lo ^ 1;
         const bool finalize = owner && k >= kc0;
         const bool carry = owner && layer && k + 1 < kc1;
         const bool compute = layer && col_valid;
-        const int el = compute ? (k * nx + je) * nx + ie : 0;
+        double ev[6];  // this layer's element fields
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            ev[i] = epre[i];
+        }
+        const double mass = mpre;
+        fetch_elem(k + 1);
+        fetch_mass(k + 1);
+        // the owned node of plane k (x y z xd yd zd), kept for the update
+        double own[6];
+        // before the last barrier of the layer: copy the owned node out of
+        // plane k's slot; after it (no thread reads plane k any more): store
+        // plane k+2 there when iteration k+1 is an element layer, and fetch
+        // plane k+3 for the iteration after
+        auto advance = [&](int phase) {
+            if (phase == 0) {
+                if (finalize) {
+#pragma unroll
+                    for (int fld = 0; fld < 6; ++fld) {
+                        own[fld] = plane[lo][fld][base];
+                    }
+                }
+            } else if (k + 1 < kc1 && k + 1 < nx) {
+                store_plane(lo);
+                if (k + 2 < kc1 && k + 2 < nx) {
+                    fetch_plane(k + 3);
+                }
+            }
+        };
 
         double F[6];
         // element corner coordinates from the LDS planes, read where needed
@@ -218,7 +280,7 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
         if (compute) {
             double x[8], y[8], z[8], pf[3][8];
             coords(x, y, z);
-            const double sig = -f.p[el] - f.q[el];
+            const double sig = -ev[0] - ev[1];
             const double det = jacobian_volume(x, y, z);
             if (det <= 0.0) {
                 atomicCAS(&s->error, 0, -1);
@@ -232,16 +294,16 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
                 }
             }
         }
-        fold(finalize, carry, layer, F, P);
+        fold(finalize, carry, layer, F, P, !hourglass, advance);
 
         // ---- hourglass control (:1291-1473, :1516-1547) ----
         if (hourglass) {
             if (compute) {
-                const double vv = f.v[el];
+                const double vv = ev[2];
                 if (vv <= 0.0) {
                     atomicCAS(&s->error, 0, -1);
                 }
-                const double determ = f.volo[el] * vv;
+                const double determ = ev[3] * vv;
                 const double volinv = 1.0 / determ;
                 double hourgam[8][4];
                 {
@@ -282,7 +344,7 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
                 }
                 const double volume13 = lulesh_cbrt(determ);
                 const double coefficient =
-                    -prm.hgcoef * 0.01 * f.ss[el] * f.elem_mass[el] / volume13;
+                    -prm.hgcoef * 0.01 * ev[4] * ev[5] / volume13;
 #pragma unroll
                 for (int t = 0; t < 3; ++t) {
                     double u[8];
@@ -307,7 +369,7 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
                     }
                 }
             }
-            fold(finalize, carry, layer, F + 3, P + 3);
+            fold(finalize, carry, layer, F + 3, P + 3, true, advance);
         }
 
         // ---- acceleration, symmetry planes, velocity, position (:1613-1683)
@@ -319,25 +381,24 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
                 fz += F[5];
             }
             const int n = (k * en + je) * en + ie;
-            const double mass = f.nodal_mass[n];
             double xdd = fx / mass;
             double ydd = fy / mass;
             double zdd = fz / mass;
             if (ie == 0) xdd = 0.0;
             if (je == 0) ydd = 0.0;
             if (k == 0) zdd = 0.0;
-            double xd = plane[lo][3][base] + xdd * dt;
+            double xd = own[3] + xdd * dt;
             if (fabs(xd) < prm.u_cut) xd = 0.0;
-            double yd = plane[lo][4][base] + ydd * dt;
+            double yd = own[4] + ydd * dt;
             if (fabs(yd) < prm.u_cut) yd = 0.0;
-            double zd = plane[lo][5][base] + zdd * dt;
+            double zd = own[5] + zdd * dt;
             if (fabs(zd) < prm.u_cut) zd = 0.0;
             f.xd[par][n] = xd;
             f.yd[par][n] = yd;
             f.zd[par][n] = zd;
-            f.x[par][n] = plane[lo][0][base] + xd * dt;
-            f.y[par][n] = plane[lo][1][base] + yd * dt;
-            f.z[par][n] = plane[lo][2][base] + zd * dt;
+            f.x[par][n] = own[0] + xd * dt;
+            f.y[par][n] = own[1] + yd * dt;
+            f.z[par][n] = own[2] + zd * dt;
         }
     }
 }

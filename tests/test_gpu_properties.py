@@ -104,3 +104,24 @@ def test_tangled_mesh_volume_error(lulesh_gpu):
     rc, _ = d.step(1, raise_on_error=False)
     assert rc == lulesh_gpu.VOLUME_ERROR
     d.close()
+
+
+def test_launch_schedule_does_not_change_results(lulesh_gpu):
+    """The sweep's launch shape and the Q/EOS band interleave only move work
+    between blocks: every choice gives the same bits."""
+    runs = {}
+    for shape, bands in (((32, 8, 16, 2), 3), ((32, 8, 5, 2), 1), ((16, 16, 7, 2), 4),
+                         ((64, 4, 3, 1), 7), ((32, 16, 9, 2), 2)):
+        d = lulesh_gpu.Domain(26, ic_mode=2, num_reg=11, cost=2, max_cycles=60)
+        d.set_sweep_shape(*shape)
+        d.set_eos_bands(bands)
+        assert d.run() == (0, 60)
+        runs[(shape, bands)] = {f: d.get(f) for f in ("x", "xd", "e", "p", "q", "v", "ss")}
+        d.close()
+    ref = next(iter(runs.values()))
+    for key, r in runs.items():
+        for f in r:
+            assert np.array_equal(bits(r[f]), bits(ref[f])), (key, f)
+    d = lulesh_gpu.Domain(5)
+    with pytest.raises(lulesh_gpu.LuleshError):
+        d.set_eos_bands(0)
