@@ -189,7 +189,30 @@ __device__ __forceinline__ void sweep_tile_mixed(const DevFieldsMixed &f, DevSca
     const int nodeoff[4] = {base, base + 1, base + PX + 1, base + PX};
 
     float P[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    // element fields of the next layer and the owned node's mass of the next
+    // plane, loaded one iteration ahead (see sweep_tile)
+    float pq_pre[3] = {0.f, 0.f, 0.f};        // p, q, ss
+    double vm_pre[3] = {0.0, 0.0, 0.0};       // v, volo, elemMass
+    double mass_pre = 1.0;
+    auto fetch_elem = [&](int kk) {
+        if (kk < nx && col_valid) {
+            const int e = (kk * nx + je) * nx + ie;
+            pq_pre[0] = f.p[e];
+            pq_pre[1] = f.q[e];
+            pq_pre[2] = f.ss[e];
+            vm_pre[0] = f.v[e];
+            vm_pre[1] = f.volo[e];
+            vm_pre[2] = f.elem_mass[e];
+        }
+    };
+    auto fetch_mass = [&](int kk) {
+        if (owner && kk >= kc0 && kk < kc1) {
+            mass_pre = f.nodal_mass[(kk * en + je) * en + ie];
+        }
+    };
     const int kbeg = (kc0 > 0) ? kc0 - 1 : 0;
+    fetch_elem(kbeg);
+    fetch_mass(kbeg);
     fetch_plane(kbeg);
     store_plane(kbeg & 1);
     if (kbeg < nx) {
@@ -210,7 +233,11 @@ __device__ __forceinline__ void sweep_tile_mixed(const DevFieldsMixed &f, DevSca
         const bool finalize = owner && k >= kc0;
         const bool carry = owner && layer && k + 1 < kc1;
         const bool compute = layer && col_valid;
-        const int el = compute ? (k * nx + je) * nx + ie : 0;
+        const float ep = pq_pre[0], eq = pq_pre[1], ess = pq_pre[2];
+        const double ev = vm_pre[0], evolo = vm_pre[1], emass = vm_pre[2];
+        const double nmass = mass_pre;
+        fetch_elem(k + 1);
+        fetch_mass(k + 1);
 
         float F[6];
         // float64 corner coordinates of component t from the LDS planes
@@ -244,7 +271,7 @@ __device__ __forceinline__ void sweep_tile_mixed(const DevFieldsMixed &f, DevSca
                 local8(Y, y);
                 local8(Z, z);
             }
-            const float sig = -f.p[el] - f.q[el];
+            const float sig = -ep - eq;
             node_normals(pf[0], pf[1], pf[2], x, y, z);
 #pragma unroll
             for (int t = 0; t < 3; ++t) {
@@ -259,11 +286,11 @@ __device__ __forceinline__ void sweep_tile_mixed(const DevFieldsMixed &f, DevSca
         // ---- hourglass control ----
         if (hourglass) {
             if (compute) {
-                const double vv = f.v[el];
+                const double vv = ev;
                 if (vv <= 0.0) {
                     raise_error(s, -1);
                 }
-                const float determ = float(f.volo[el] * vv);
+                const float determ = float(evolo * vv);
                 const float volinv = frcp(determ);
                 float hourgam[8][4];
                 {
@@ -308,7 +335,7 @@ __device__ __forceinline__ void sweep_tile_mixed(const DevFieldsMixed &f, DevSca
                     }
                 }
                 const float coefficient =
-                    hg * f.ss[el] * float(f.elem_mass[el]) * frcp(fcbrt(determ));
+                    hg * ess * float(emass) * frcp(fcbrt(determ));
 #pragma unroll
                 for (int t = 0; t < 3; ++t) {
                     float u[8];
@@ -347,7 +374,7 @@ __device__ __forceinline__ void sweep_tile_mixed(const DevFieldsMixed &f, DevSca
                 fz += double(F[5]);
             }
             const int n = (k * en + je) * en + ie;
-            const double minv = 1.0 / f.nodal_mass[n];
+            const double minv = 1.0 / nmass;
             double xdd = fx * minv;
             double ydd = fy * minv;
             double zdd = fz * minv;
