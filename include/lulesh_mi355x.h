@@ -229,8 +229,9 @@ double lulesh_last_step_ms(lulesh_domain *d);
 /* Message of the last error on this domain ("" when none). */
 const char *lulesh_last_error(const lulesh_domain *d);
 
-/* The library's cube root (lulesh_cbrt.h) evaluated on the GPU, and IEEE
- * sqrt / division, for n inputs: a math-parity probe for the tests. */
+/* The library's cube root (lulesh_cbrt.h) evaluated on the GPU (op 0), IEEE
+ * sqrt (1) and division (2), and the EOS's square root for inputs >= 2^-767
+ * (3), for n inputs: a math-parity probe for the tests. */
 int lulesh_device_math(int op, int n, const double *a, const double *b,
                        double *out);
 

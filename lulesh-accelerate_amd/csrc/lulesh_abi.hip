@@ -1083,7 +1083,7 @@ const char *lulesh_last_error(const lulesh_domain *d)
 int lulesh_device_math(int op, int n, const double *a, const double *b,
                        double *out)
 {
-    if (n < 0 || !a || !out || (op == 2 && !b) || op < 0 || op > 2) {
+    if (n < 0 || !a || !out || (op == 2 && !b) || op < 0 || op > 3) {
         return LULESH_BAD_ARG;
     }
     if (n == 0) {

@@ -268,6 +268,29 @@ struct EosOut {
     double p, e, q, ss;
 };
 
+// Correctly rounded square root of s >= 2^-767 (and of +inf and NaN): the
+// Newton-Raphson refinement of the hardware reciprocal square root that the
+// compiler emits for sqrt(double) on gfx950 (two coupled iterations on the
+// root g*s and the half-reciprocal g/2, then two residual corrections
+// d = s - r*r, r += d*h), without the input scaling it adds for inputs
+// below 2^-767 and without its zero test.  For such s it is sqrt(s) bit for
+// bit (checked against the host's sqrt in the tests); the EOS calls it only
+// above its sound-speed floor (>= 1e-37).
+__device__ __forceinline__ double sqrt_above_tiny(double s)
+{
+    const double g = __builtin_amdgcn_rsq(s);
+    double r = s * g;
+    double h = g * 0.5;
+    const double e = __builtin_fma(-h, r, 0.5);
+    r = __builtin_fma(r, e, r);
+    h = __builtin_fma(h, e, h);
+    double d = __builtin_fma(-r, r, s);
+    r = __builtin_fma(d, h, r);
+    d = __builtin_fma(-r, r, s);
+    r = __builtin_fma(d, h, r);
+    return (s == __builtin_inf()) ? s : r;
+}
+
 // the sound speed squared over rho0; UnitRho0 (rho0 == 1, where x / rho0
 // is x exactly) leaves the divide out at compile time
 template <bool UnitRho0>
@@ -282,7 +305,7 @@ __device__ __forceinline__ double eos_sound(double pbvc, double e, double vv,
     if (s <= tiny) {
         s = floor_v;
     } else {
-        s = sqrt(s);
+        s = sqrt_above_tiny(s);
     }
     return s;
 }

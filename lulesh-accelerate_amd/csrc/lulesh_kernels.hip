@@ -898,7 +898,8 @@ __global__ void k_dt_finish_batch(const BatchMember *members, BatchWork *work,
     }
 }
 
-// device math probe: op 0 = lulesh_cbrt(a), 1 = sqrt(a), 2 = a / b
+// device math probe: op 0 = lulesh_cbrt(a), 1 = sqrt(a), 2 = a / b,
+// 3 = sqrt_above_tiny(a) (the EOS's square root)
 __global__ void k_device_math(int op, int n, const double *a, const double *b,
                               double *out)
 {
@@ -910,6 +911,8 @@ __global__ void k_device_math(int op, int n, const double *a, const double *b,
         out[t] = lulesh_cbrt(a[t]);
     } else if (op == 1) {
         out[t] = sqrt(a[t]);
+    } else if (op == 3) {
+        out[t] = sqrt_above_tiny(a[t]);
     } else {
         out[t] = a[t] / b[t];
     }

@@ -497,8 +497,9 @@ class Domain:
 
 
 def device_math(op: str, a, b=None) -> np.ndarray:
-    """Evaluates the library's cube root, sqrt or division on the GPU."""
-    code = {"cbrt": 0, "sqrt": 1, "div": 2}[op]
+    """Evaluates the library's cube root, sqrt, division or the EOS's square
+    root (inputs >= 2^-767) on the GPU."""
+    code = {"cbrt": 0, "sqrt": 1, "div": 2, "sqrt_eos": 3}[op]
     a = np.ascontiguousarray(a, np.float64)
     bb = np.ascontiguousarray(b, np.float64) if b is not None else None
     out = np.empty_like(a)

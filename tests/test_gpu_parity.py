@@ -68,6 +68,18 @@ def test_device_math_matches_host(lulesh_gpu, oracle):
     assert bits_equal(lulesh_gpu.device_math("div", a, b), a / b)
 
 
+def test_eos_sqrt_is_sqrt(lulesh_gpu):
+    """The EOS's square root skips the scaling for tiny inputs: above 2^-767
+    it is IEEE sqrt bit for bit, over the whole exponent range the EOS can
+    see (its sound-speed floor is 1.1e-37) and at the edges."""
+    rng = np.random.default_rng(7)
+    a = np.ldexp(1 + rng.uniform(0, 1, 400000), rng.integers(-766, 1023, 400000))
+    a = np.concatenate([a, np.ldexp(1.0, np.arange(-767, 1024)), [1.1111111e-37,
+                        np.nextafter(np.ldexp(1.0, -767), 1.0), np.finfo(np.float64).max,
+                        np.inf, 1.0, 4.0, 2.0]])
+    assert bits_equal(lulesh_gpu.device_math("sqrt_eos", a), np.sqrt(a))
+
+
 @pytest.mark.parametrize("mode", [1, 2])
 def test_first_cycles_bitwise(lulesh_gpu, oracle, mode):
     d, o = make_pair(lulesh_gpu, oracle, mode, 7)
