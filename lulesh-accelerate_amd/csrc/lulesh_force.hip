@@ -44,7 +44,17 @@ constexpr int kStencil[8][6] = {{1, 2, 3, 4, 5, 7}, {2, 3, 0, 5, 6, 4},
 // One block's share of the sweep: node columns [bx*(TX-1), (bx+1)*(TX-1)) x
 // [by*(TY-1), (by+1)*(TY-1)) and node planes [bz*nz, (bz+1)*nz) of one
 // domain.
-template <int TX, int TY>
+//
+// Fast (the single domain's kernel): the element fields of the next layer
+// and the owned node's mass of the next plane are loaded one layer ahead
+// (the barriers keep the compiler from moving loads up, so a load issued in
+// the layer that uses it waits for HBM), and the owned node is copied to
+// registers so that the next node plane is stored inside the layer's last
+// fold: four barriers per layer.  The ensemble's tile loop has no registers
+// to spare for these (they spill there), so its kernel stores each plane at
+// the top of the layer (six barriers) and loads the fields where they are
+// used.  Both give the same bits.
+template <int TX, int TY, bool Fast>
 __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
                                            const PhysParams &prm, int nz, int bx,
                                            int by, int bz)
@@ -151,7 +161,7 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
     // slot takes plane k+2 after it.
     auto fold = [&](bool finalize, bool carry, bool layer, double *F, double *P, bool last,
                     auto &&advance) {
-        if (last) {
+        if (Fast && last) {
             advance(0);
         }
         __syncthreads();
@@ -176,7 +186,7 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
                 P[t] = acc;
             }
         }
-        if (last) {
+        if (Fast && last) {
             advance(1);
         }
         __syncthreads();
@@ -212,33 +222,63 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
     // (k < nx), k+1 (the other slot); both are in LDS when it starts, and
     // plane k+2 waits in registers
     const int kbeg = (kc0 > 0) ? kc0 - 1 : 0;
-    fetch_elem(kbeg);
-    fetch_mass(kbeg);
+    if constexpr (Fast) {
+        fetch_elem(kbeg);
+        fetch_mass(kbeg);
+    }
     fetch_plane(kbeg);
     store_plane(kbeg & 1);
     if (kbeg < nx) {
         fetch_plane(kbeg + 1);
-        store_plane((kbeg + 1) & 1);
-        if (kbeg + 1 < nx && kbeg + 1 < kc1) {
-            fetch_plane(kbeg + 2);
+        if constexpr (Fast) {
+            store_plane((kbeg + 1) & 1);
+            if (kbeg + 1 < nx && kbeg + 1 < kc1) {
+                fetch_plane(kbeg + 2);
+            }
         }
     }
-    __syncthreads();
+    if constexpr (Fast) {
+        __syncthreads();
+    }
     for (int k = kbeg; k < kc1; ++k) {
         const bool layer = k < nx;
+        if constexpr (!Fast) {
+            __syncthreads();  // the readers of the slot below are done
+            if (layer) {
+                store_plane((k + 1) & 1);  // plane k+1, fetched one layer ago
+            }
+            __syncthreads();
+            if (layer && k + 1 < nx && k + 1 < kc1) {
+                fetch_plane(k + 2);  // in flight while this layer computes
+            }
+        }
         const int lo = k & 1;
         const int hi = lo ^ 1;
         const bool finalize = owner && k >= kc0;
         const bool carry = owner && layer && k + 1 < kc1;
         const bool compute = layer && col_valid;
-        double ev[6];  // this layer's element fields
+        // this layer's element fields (p q v volo ss elemMass) and the owned
+        // node's mass: prefetched (Fast), else loaded where they are used
+        double ev[6];
+        double mass = 1.0;
+        if constexpr (Fast) {
 #pragma unroll
-        for (int i = 0; i < 6; ++i) {
-            ev[i] = epre[i];
+            for (int i = 0; i < 6; ++i) {
+                ev[i] = epre[i];
+            }
+            mass = mpre;
+            fetch_elem(k + 1);
+            fetch_mass(k + 1);
         }
-        const double mass = mpre;
-        fetch_elem(k + 1);
-        fetch_mass(k + 1);
+        const int el = compute ? (k * nx + je) * nx + ie : 0;
+        auto field = [&](int i) -> double {
+            if constexpr (Fast) {
+                return ev[i];
+            } else {
+                const double *const a[6] = {f.p, f.q, f.v, f.volo, f.ss, f.elem_mass};
+                return a[i][el];
+            }
+        };
         // the owned node of plane k (x y z xd yd zd), kept for the update
         double own[6];
         // before the last barrier of the layer: copy the owned node out of
@@ -280,7 +320,7 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
         if (compute) {
             double x[8], y[8], z[8], pf[3][8];
             coords(x, y, z);
-            const double sig = -ev[0] - ev[1];
+            const double sig = -field(0) - field(1);
             const double det = jacobian_volume(x, y, z);
             if (det <= 0.0) {
                 atomicCAS(&s->error, 0, -1);
@@ -299,11 +339,11 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
         // ---- hourglass control (:1291-1473, :1516-1547) ----
         if (hourglass) {
             if (compute) {
-                const double vv = ev[2];
+                const double vv = field(2);
                 if (vv <= 0.0) {
                     atomicCAS(&s->error, 0, -1);
                 }
-                const double determ = ev[3] * vv;
+                const double determ = field(3) * vv;
                 const double volinv = 1.0 / determ;
                 double hourgam[8][4];
                 {
@@ -344,7 +384,7 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
                 }
                 const double volume13 = lulesh_cbrt(determ);
                 const double coefficient =
-                    -prm.hgcoef * 0.01 * ev[4] * ev[5] / volume13;
+                    -prm.hgcoef * 0.01 * field(4) * field(5) / volume13;
 #pragma unroll
                 for (int t = 0; t < 3; ++t) {
                     double u[8];
@@ -381,6 +421,13 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
                 fz += F[5];
             }
             const int n = (k * en + je) * en + ie;
+            if constexpr (!Fast) {
+                mass = f.nodal_mass[n];
+#pragma unroll
+                for (int fld = 0; fld < 6; ++fld) {
+                    own[fld] = plane[lo][fld][base];
+                }
+            }
             double xdd = fx / mass;
             double ydd = fy / mass;
             double zdd = fz / mass;
@@ -407,8 +454,8 @@ template <int TX, int TY, int W>
 __global__ __launch_bounds__(TX *TY, W) void k_force_sweep(DevFields f, DevScalars *s,
                                                         PhysParams prm, int nz)
 {
-    sweep_tile<TX, TY>(f, s, prm, nz, int(blockIdx.x), int(blockIdx.y),
-                       int(blockIdx.z));
+    sweep_tile<TX, TY, true>(f, s, prm, nz, int(blockIdx.x), int(blockIdx.y),
+                             int(blockIdx.z));
 }
 
 // The ensemble form: a persistent grid walks the tile list of all members;
@@ -423,7 +470,7 @@ __global__ __launch_bounds__(TX *TY, W) void k_force_sweep_batch(
         const int4 w = work->tiles[t];
         __syncthreads();  // the LDS of the previous tile is no longer read
         const BatchMember &m = members[w.x];
-        sweep_tile<TX, TY>(m.f, m.s, prm, nz, w.y, w.z, w.w);
+        sweep_tile<TX, TY, false>(m.f, m.s, prm, nz, w.y, w.z, w.w);
     }
 }
 
