@@ -307,7 +307,7 @@ def bench_learned_q(lulesh_amd, efactor: float = 1.0) -> dict:
     """The learned-Q form: the network shipped with the package
     (lulesh-accelerate_amd/learned_q_net.npz, trained by learned_q.py on
     fine 120^3 runs of other energy factors) at 40^3, beside the plain
-    limiter at 40^3, both judged by the 120^3 run of the same factor
+    limiter at 40^3, both measured against the 120^3 run of the same factor
     restricted to 40^3 at t = 1e-2."""
     import learned_q
     net, meta = learned_q.load()

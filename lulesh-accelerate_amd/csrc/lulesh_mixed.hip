@@ -786,11 +786,12 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_mixed(DevFieldsMixed f, DevSca
         EosOutMixed o;
         o.e = 0.0;
         o.p = o.q = o.ss = 0.f;
-        // every repetition of the region is evaluated (see k_q_eos_dt)
-#pragma unroll 2
+        // every repetition of the region is evaluated: the inputs pass through
+        // an empty asm statement at the top of each one, carried in the same
+        // registers (see k_q_eos_dt)
+        double a0 = vnewc, a1 = e_old;
+        float a2 = delvc, a3 = p_old, a4 = q_old, a5 = qquad, a6 = qlin;
         for (int r = 0; r < rep; ++r) {
-            double a0 = vnewc, a1 = e_old;
-            float a2 = delvc, a3 = p_old, a4 = q_old, a5 = qquad, a6 = qlin;
             asm volatile("" : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(a4),
                          "+v"(a5), "+v"(a6));
             o = eos_element_mixed(a0, a1, a2, a3, a4, a5, a6, ec);

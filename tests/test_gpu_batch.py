@@ -113,7 +113,7 @@ def test_members_stop_on_their_own(lulesh_gpu):
     assert len(set(cycles)) == 3
 
 
-def test_config3_ensemble_members(lulesh_gpu, oracle):
+def test_default_ensemble_members(lulesh_gpu, oracle):
     """The default 64-member ensemble (nx in [16, 64], factors in
     [0.8, 1.2], seed 0) to t = 1e-2; its smallest member against the oracle
     and every member's stop time reached."""

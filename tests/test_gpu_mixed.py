@@ -1,6 +1,6 @@
 """The mixed-precision form (lulesh_create_mixed): float32 element-local
 arithmetic with float64 nodes, volumes, energy update and dt reductions.
-Judged against the float64 step (itself bit-identical to the oracle) on the
+Measured against the float64 step (itself bit-identical to the oracle) on the
 quantities the form promises: origin energy within 1e-5 relative at
 t = 1e-2, total energy, shock position, and the cycle count.
 """

@@ -5,7 +5,7 @@ is compared bit for bit with the oracle run with that cube root
 (cbrt_mode=1), and within tolerance with the reference itself (golden dumps
 and published tables, which use the C library's cbrt).
 
-Tolerances against the reference (north-star bounds): same cycle count,
+Tolerances against the reference: same cycle count,
 origin energy within 1e-10 relative, plane-0 symmetry MaxRelDiff <= 1e-12
 in 1.0 mode (2.0 mode: within 1e-12 of the reference's own value, which is
 1.461e-12 at 30^3), node and element fields within 1e-10 relative
@@ -176,9 +176,9 @@ def test_known_answers(lulesh_gpu, mode, row):
 
 @pytest.mark.parametrize("mode", [1, 2])
 def test_30_cubed_against_oracle(lulesh_gpu, oracle, mode):
-    """The full 30^3 run (config 1 of the benchmark set): bitwise against the
-    oracle with the project cube root, and the north-star bounds against
-    the oracle with the C library's cube root (= the reference)."""
+    """The full 30^3 run (the reference's CPU-sized case): bitwise against
+    the oracle with the project cube root, and the reference tolerances
+    against the oracle with the C library's cube root (= the reference)."""
     d, o = make_pair(lulesh_gpu, oracle, mode, 30)
     assert d.run()[0] == 0
     assert o.run() == 0
