@@ -34,6 +34,8 @@ import subprocess
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "lulesh-accelerate_amd"))
 
@@ -313,13 +315,20 @@ def bench_learned_q(lulesh_amd, efactor: float = 1.0) -> dict:
     net, meta = learned_q.load()
     res = learned_q.evaluate(net, [efactor])[f"{efactor:.4f}"]
     plain, learned = res["plain"], res["learned"]
+    eq = res["learned_equivalent_plain"]
+    cost = [eq[k]["cost_ratio"] for k in learned_q.METRICS]
     return {
         "value": learned["zones_cycles_per_s"], "unit": "zones·cycles/s",
         "workload": "sedov 40^3 (LULESH 2.0 ICs, 11 regions) to t=1e-2, learned "
                     "closure vs plain limiter, errors against the 120^3 run "
                     "restricted to 40^3",
-        "plain_40": plain, "learned_40": learned,
+        "plain_40": plain, "learned_40": learned, "plain_80": res["plain_80"],
         "fine_120_cycles": res["fine_cycles"],
+        # the plain resolution whose error the learned 40^3 run matches (a
+        # power law through the plain errors at 40^3 and 80^3) and the
+        # zone*cycles that plain run would cost relative to 40^3
+        "learned_equivalent_plain": eq,
+        "equivalent_cost_ratio_geomean": float(np.exp(np.mean(np.log(cost)))),
         "error_ratio_learned_over_plain": {
             k: learned[k] / plain[k] for k in ("density_l1", "energy_l1", "pressure_l1")},
         "training": {k: meta[k] for k in ("samples", "epochs", "loss_init", "loss_final",

@@ -232,12 +232,17 @@ def load(path: str = NET_FILE):
     return net, meta
 
 
-def errors(coarse: "lulesh_amd.Domain", ref: dict) -> dict:
-    """L1 errors of a coarse solution against a restricted fine one."""
-    rho_c, rho_f = 1.0 / coarse.get("v"), 1.0 / ref["v"]
-    e_c, e_f = coarse.get("e"), ref["e"]
-    p_c, p_f = coarse.get("p"), ref["p"]
-    n = coarse.nx
+METRICS = ("density_l1", "energy_l1", "pressure_l1")
+
+
+def errors(coarse, ref: dict) -> dict:
+    """L1 errors of a coarse solution (a Domain, or a restricted state dict
+    with v, e, p) against a restricted fine one."""
+    get = coarse.get if hasattr(coarse, "nx") else coarse.__getitem__
+    rho_c, rho_f = 1.0 / get("v"), 1.0 / ref["v"]
+    e_c, e_f = get("e"), ref["e"]
+    p_c, p_f = get("p"), ref["p"]
+    n = round(len(e_f) ** (1.0 / 3.0))
     d = {
         "density_l1": float(np.abs(rho_c - rho_f).sum() / np.abs(rho_f - 1.0).sum()),
         "energy_l1": float(np.abs(e_c - e_f).sum() / np.abs(e_f).sum()),
@@ -249,7 +254,27 @@ def errors(coarse: "lulesh_amd.Domain", ref: dict) -> dict:
     return d
 
 
-def evaluate(net, efactors, fine_nx=120, ratio=3, z_max=4.0) -> dict:
+def equivalent_resolution(nc: int, plain: dict, mid: int, plain_mid: dict,
+                          target: dict) -> dict:
+    """The plain-limiter resolution that would reach the target errors: a
+    power law err(N) = a N^-s through the plain errors at nc and mid (both
+    measured on the nc^3 grid), solved for the target's error, per metric,
+    with the zone*cycle cost of that run relative to nc^3 (cycles grow
+    with N, so the cost goes as N^4).  An estimate from two points."""
+    out = {}
+    for k in METRICS:
+        s_k = np.log(plain[k] / plain_mid[k]) / np.log(mid / nc)
+        n_eq = nc * (plain[k] / target[k]) ** (1.0 / s_k)
+        out[k] = {"order": float(s_k), "plain_nx": float(n_eq),
+                  "cost_ratio": float((n_eq / nc) ** 4)}
+    return out
+
+
+def evaluate(net, efactors, fine_nx=120, ratio=3, z_max=4.0, mid_ratio=2) -> dict:
+    """Errors of the plain and learned limiters at fine_nx / ratio against
+    the fine run restricted to that grid, and of the plain limiter at
+    mid_ratio times that resolution (restricted the same way), from which
+    the plain resolution matching the learned errors is estimated."""
     out = {}
     nc = fine_nx // ratio
     for f in efactors:
@@ -273,6 +298,19 @@ def evaluate(net, efactors, fine_nx=120, ratio=3, z_max=4.0) -> dict:
             row[name].update({"cycles": st.cycle, "rc": rc,
                               "zones_cycles_per_s": nc ** 3 * n / wall})
             d.close()
+        if mid_ratio:
+            nm = nc * mid_ratio
+            d = lulesh_amd.Domain(nm, efactor=float(f))
+            d.step(1)
+            t1 = time.perf_counter()
+            rc, n = d.run()
+            wall = time.perf_counter() - t1
+            row[f"plain_{nm}"] = errors(restrict(d, mid_ratio), ref)
+            row[f"plain_{nm}"].update({"cycles": d.stats().cycle, "rc": rc,
+                                      "zones_cycles_per_s": nm ** 3 * n / wall})
+            d.close()
+            row["learned_equivalent_plain"] = equivalent_resolution(
+                nc, row["plain"], nm, row[f"plain_{nm}"], row["learned"])
         out[f"{float(f):.4f}"] = row
         log(f"learned_q: efactor {f:.4f}: plain {row['plain']['density_l1']:.4f} "
             f"learned {row['learned']['density_l1']:.4f} (density L1), "
