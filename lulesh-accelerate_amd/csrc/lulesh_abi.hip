@@ -40,8 +40,9 @@ const char *const kKernelNames[KID_COUNT] = {
     "k_dt_finish", "k_corner_forces_u", "k_nodes_u",
 };
 
-// default launch shape of the force sweep (32 x 8 element columns, 16 node
-// planes per block, register budget for 2 waves per SIMD);
+// default launch shape of the force sweep (32 x 8 element columns, at most
+// 16 node planes per block, register budget for 2 waves per SIMD; the planes
+// per block shrink on small domains, see default_sweep_planes);
 // lulesh_set_sweep_shape overrides it
 constexpr SweepShape kDefaultSweep = {32, 8, 16, 2};
 // the mixed-precision sweep holds float32 corner forces: three blocks fit
@@ -57,6 +58,34 @@ constexpr int kDefaultEosBands = 3;
 
 constexpr int kBlock = 256;
 constexpr int kBatch = 32;  // cycles queued per host round trip in run()
+
+// Node planes per block of the default sweep on an nx^3 domain: the largest
+// nz <= 16 whose grid still fills 90% of the resident block slots (two
+// 256-thread blocks per CU).  A block sweeps its planes one after another,
+// so on small domains parallelism comes from cutting the columns into
+// short runs: at 30^3 nz = 1 takes the cycle from 138 to 45 us, at 64^3
+// nz = 4 from 162 to 95 us; from 120^3 on nz = 16 is kept (measured on
+// MI355X).
+int default_sweep_planes(int nx)
+{
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+            hipSuccess ||
+        cus < 1) {
+        cus = 256;
+    }
+    const long long want = (long long)(0.9 * 2 * cus);
+    const int en = nx + 1;
+    const long long cols = (long long)((en + kDefaultSweep.tx - 2) / (kDefaultSweep.tx - 1)) *
+                           ((en + kDefaultSweep.ty - 2) / (kDefaultSweep.ty - 1));
+    for (int nz = kDefaultSweep.nz; nz > 1; --nz) {
+        if (cols * ((en + nz - 1) / nz) >= want) {
+            return nz;
+        }
+    }
+    return 1;
+}
 
 struct TimedLaunch {
     int kernel;
@@ -383,6 +412,9 @@ int create_domain(const lulesh_config *cfg, bool mixed, double efactor,
     d->cfg = *cfg;
     d->mixed = mixed;
     d->sweep = mixed ? kDefaultSweepMixed : kDefaultSweep;
+    if (!mixed && cfg->nx >= 1 && cfg->nx <= 600) {
+        d->sweep.nz = default_sweep_planes(cfg->nx);
+    }
     HostMesh m;
     try {
         MeshConfig mc;
