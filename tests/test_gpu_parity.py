@@ -157,9 +157,8 @@ def test_matches_reference_dumps_within_tolerance(lulesh_gpu, name):
 KNOWN = json.load(open(os.path.join(GOLDEN, "known_answers.json")))
 
 
-@pytest.mark.parametrize("mode,row", [
-    (1, r) for r in KNOWN["ic_1_0"] if r["nx"] <= 45] + [
-    (2, r) for r in KNOWN["ic_2_0"] if r["nx"] <= 45])
+@pytest.mark.parametrize("mode,row", [(1, r) for r in KNOWN["ic_1_0"]] +
+                         [(2, r) for r in KNOWN["ic_2_0"]])
 def test_known_answers(lulesh_gpu, mode, row):
     d = lulesh_gpu.Domain(row["nx"], ic_mode=mode)
     rc, n = d.run()
