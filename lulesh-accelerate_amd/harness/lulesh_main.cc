@@ -31,7 +31,10 @@ void usage(const char *prog)
         " -r <numregions> : Number of distinct regions (def: 11)\n"
         " -b <balance>    : Load balance between regions of a domain (def: 1)\n"
         " -c <cost>       : Extra cost of more expensive regions (def: 1)\n"
+        " -f <numfiles>   : Number of files to split viz dump into (def: 1; one\n"
+        "                   VTK file is written whatever the count)\n"
         " -p              : Print out progress\n"
+        " -v              : Output viz file: lulesh_plot_c<cycle>.vtk\n"
         " --ic <1|2>      : LULESH 1.0 or 2.0 initial conditions (def: 2)\n"
         " -u <file>       : unstructured mesh from a .lmesh file (LULESH 1.0)\n"
         " --vtk <file>    : write the final state as a VTK file\n"
@@ -62,7 +65,8 @@ int main(int argc, char **argv)
 {
     lulesh_config cfg;
     lulesh_config_default(&cfg);
-    bool quiet = false, progress = false;
+    bool quiet = false, progress = false, viz = false;
+    int num_files = 1;
     std::string lmesh, vtk;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
@@ -84,6 +88,14 @@ int main(int argc, char **argv)
                 return 1;
             }
             (a == "-u" ? lmesh : vtk) = argv[++i];
+        } else if (a == "-f") {
+            if (!int_arg(argc, argv, i, num_files)) return 1;
+            if (num_files < 1) {
+                std::fprintf(stderr, "-f needs a positive number of files\n");
+                return 1;
+            }
+        } else if (a == "-v") {
+            viz = true;
         } else if (a == "-p") {
             progress = true;
         } else if (a == "-q") {
@@ -154,6 +166,10 @@ int main(int argc, char **argv)
     double mabs = 0, tabs = 0, mrel = 0;
     lulesh_symmetry(d, &mabs, &tabs, &mrel);  // the cube only
     int status = 0;
+    if (viz && vtk.empty()) {
+        // the reference's dump name (lulesh-2.0.3/lulesh-viz.cc:62)
+        vtk = "lulesh_plot_c" + std::to_string(st.cycle) + ".vtk";
+    }
     if (!vtk.empty() && lulesh_write_domain(d, vtk.c_str()) != LULESH_OK) {
         std::fprintf(stderr, "writing %s failed\n", vtk.c_str());
         status = 1;

@@ -13,9 +13,9 @@ HARNESS = os.path.join(ROOT, "lulesh-accelerate_amd", "lulesh_mi355x")
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 
-def run(args):
+def run(args, cwd=None):
     out = subprocess.run([HARNESS] + args, capture_output=True, text=True,
-                         timeout=300)
+                         timeout=300, cwd=cwd)
     assert out.returncode == 0, out.stderr
     return out.stdout
 
@@ -54,3 +54,10 @@ def test_harness_unstructured_mesh(tmp_path):
     e0 = lulesh_oracle.lib().lo_scalar(o.h, b"e_origin")
     assert f"{res['origin_energy']:12.6e}" == f"{e0:12.6e}"
     assert os.path.getsize(vtk) > 0
+
+
+def test_harness_viz_dump(tmp_path):
+    """-v writes the reference's dump name for the final cycle (-f is taken
+    and one file written)."""
+    run(["-s", "6", "-i", "12", "-v", "-f", "4", "-q"], cwd=str(tmp_path))
+    assert os.path.getsize(tmp_path / "lulesh_plot_c12.vtk") > 0
