@@ -146,6 +146,26 @@ struct BatchMember {
     DevScalars *s;
 };
 
+// A member read from the work lists is the same for every lane of a wave,
+// but the compiler cannot prove it from a loaded index, so it would keep
+// every field pointer in vector registers.  Passing each 32-bit word
+// through readfirstlane marks the copy wave-uniform: the fields a kernel
+// uses then live in scalar registers.  The values are unchanged.
+__device__ __forceinline__ BatchMember uniform_member(const BatchMember &m)
+{
+    static_assert(sizeof(BatchMember) % 4 == 0, "BatchMember is a whole number of words");
+    constexpr int kWords = int(sizeof(BatchMember) / 4);
+    int w[kWords];
+    __builtin_memcpy(w, &m, sizeof(BatchMember));
+#pragma unroll
+    for (int i = 0; i < kWords; ++i) {
+        w[i] = __builtin_amdgcn_readfirstlane(w[i]);
+    }
+    BatchMember u;
+    __builtin_memcpy(&u, w, sizeof(BatchMember));
+    return u;
+}
+
 // Work lists of the persistent ensemble kernels, in device memory.  The
 // host rewrites them (between cycles) when members finish, so the kernels
 // read the counts from here rather than from their launch arguments.
