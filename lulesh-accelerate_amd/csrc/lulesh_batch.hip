@@ -641,11 +641,15 @@ int lulesh_batch_autotune(lulesh_batch *b, int trial_cycles, double *best_ms)
         *t = b->last_ms / reps;
         return LULESH_OK;
     };
-    BHIP_TRY(b, hipMemcpyAsync(save, b->arena, b->arena_bytes, hipMemcpyDeviceToDevice,
-                               b->stream));
-    BHIP_TRY(b, hipMemcpyAsync(save_s, b->d_scalars, size_t(b->n) * sizeof(DevScalars),
-                               hipMemcpyDeviceToDevice, b->stream));
-    BHIP_TRY(b, hipStreamSynchronize(b->stream));
+    if (hipMemcpyAsync(save, b->arena, b->arena_bytes, hipMemcpyDeviceToDevice,
+                       b->stream) != hipSuccess ||
+        hipMemcpyAsync(save_s, b->d_scalars, size_t(b->n) * sizeof(DevScalars),
+                       hipMemcpyDeviceToDevice, b->stream) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess) {
+        (void)hipFree(save);
+        (void)hipFree(save_s);
+        return bfail(b, LULESH_HIP_ERROR, "autotune: saving the state failed");
+    }
 
     // 1) tile shape and planes per block, one block per tile
     const SweepShape shapes[] = {{32, 8, 0, 2}, {16, 16, 0, 2}, {16, 8, 0, 2},
