@@ -125,3 +125,17 @@ def test_launch_schedule_does_not_change_results(lulesh_gpu):
     d = lulesh_gpu.Domain(5)
     with pytest.raises(lulesh_gpu.LuleshError):
         d.set_eos_bands(0)
+
+
+def test_largest_accepted_domain(lulesh_gpu):
+    """nx = 600, the largest edge the library accepts (601^3 nodes keep every
+    index in 32 bits): a few cycles run, stay finite and keep the Sedov
+    symmetry; 601 is refused."""
+    d = lulesh_gpu.Domain(600, ic_mode=2, max_cycles=3)
+    assert d.run() == (0, 3)
+    assert np.isfinite(d.get("e")).all() and np.isfinite(d.get("xd")).all()
+    # the published criterion (reference/lulesh-ports.pdf section 6.2)
+    assert d.symmetry()["max_rel_diff"] <= 1e-8
+    d.close()
+    with pytest.raises(lulesh_gpu.LuleshError):
+        lulesh_gpu.Domain(601)
