@@ -6,16 +6,21 @@
  * lulesh-2.0.3/lulesh.cc:899).  The GPU has no glibc, and glibc's cbrt is not
  * correctly rounded, so no device routine can promise its exact bits.  This
  * header defines one cube root as a fixed sequence of IEEE-754 binary64
- * operations (+ - * / and fma, each correctly rounded on x86-64 and on
+ * operations (+ - * and fma, each correctly rounded on x86-64 and on
  * gfx950), so the same source compiled for the host and for the device gives
- * identical bits.  It is accurate to about half an ulp; the tests bound its
- * distance from the C library's cbrt.
+ * identical bits.  It is correctly rounded except for roots within about
+ * 1e-10 ulp of a rounding boundary; the tests check the rounding exactly and
+ * bound the distance from the C library's cbrt.
  *
- * Method: split x = t * 2^(3q) with t in [1, 8); a cubic least-squares fit of
- * t^(1/3) (relative error < 1.4e-2), three Newton steps y <- (2y + t/y^2)/3
- * (error ~1e-2 -> 2e-4 -> 4e-8 -> 2e-15), then one Newton step whose residual
- * y^3 - t is formed with fma so that it is exact to within an ulp of the
- * residual.  The result is y * 2^q, applied to the exponent field.
+ * Method: split x = t * 2^(3q) with t = m * 2^r, m in [1, 2), r in {0, 1, 2}.
+ * The reciprocal root z ~ t^(-1/3) starts from a cubic minimax fit of
+ * m^(-1/3) on [1, 2] times 2^(-r/3) (relative error < 2.6e-4) and takes two
+ * Newton steps z <- z + z (1 - t z^3) / 3 (error -> 1.3e-7 -> 3.4e-14).  Then
+ * y = t z^2 is the root to about 7e-14, and one Newton step on y, whose
+ * residual y^3 - t is formed with fma so that it is exact to within an ulp of
+ * the residual and whose slope 1 / (3 y^2) is z^2 / 3, leaves an error near
+ * 1e-26 before the final rounding.  No step divides.  The result is y * 2^q,
+ * applied to the exponent field.
  *
  * The includer defines LULESH_FN (for example `static inline` on the host or
  * `static __device__ inline` in HIP code) before including this header.
@@ -48,22 +53,29 @@ LULESH_FN double lulesh_cbrt_pos_normal(double x)
 {
     const uint64_t b = lulesh_double_to_bits(x);
     const int e = (int)((b >> 52) & 0x7ffu) - 1023;
-    /* q = floor(e / 3), r = e - 3q in {0, 1, 2} */
-    const int q = (e >= 0) ? (e / 3) : -((2 - e) / 3);
+    /* q = floor(e / 3) for e in [-1022, 1023]; r = e - 3q in {0, 1, 2} */
+    const int q = (int)((unsigned)(e + 1026) / 3u) - 342;
     const int r = e - 3 * q;
-    const double t = lulesh_bits_to_double((b & 0x000fffffffffffffull) |
-                                           ((uint64_t)(1023 + r) << 52));
-    double y = 0.7091350135760441 +
-               t * (0.3400187424911607 +
-                    t * (-0.037495962996826254 + t * 0.0019167009490987084));
-    y = (2.0 * y + t / (y * y)) / 3.0;
-    y = (2.0 * y + t / (y * y)) / 3.0;
-    y = (2.0 * y + t / (y * y)) / 3.0;
+    const uint64_t frac = b & 0x000fffffffffffffull;
+    const double m = lulesh_bits_to_double(frac | (1023ull << 52));
+    const double t = lulesh_bits_to_double(frac | ((uint64_t)(1023 + r) << 52));
+    const double third = 0.3333333333333333;
+    /* 2^(-r/3) */
+    const double c = (r == 0) ? 1.0 : ((r == 1) ? 0.7937005259840998 : 0.6299605249474366);
+    double z = fma(m, fma(m, fma(m, -0.04993701447458462, 0.3177574893441185),
+                          -0.809960645949834),
+                   1.5418870676993346) * c;
+    for (int i = 0; i < 2; ++i) {
+        const double d = fma(-t, (z * z) * z, 1.0); /* 1 - t z^3 */
+        z = fma(z * d, third, z);
+    }
+    const double zz = z * z;
+    double y = t * zz;
     {
         const double y2 = y * y;
         const double y2lo = fma(y, y, -y2);          /* y*y = y2 + y2lo exactly */
         const double res = fma(y, y2, -t) + y * y2lo; /* y^3 - t */
-        y = y - res / (3.0 * y2);
+        y = fma(-res, zz * third, y);
     }
     return lulesh_bits_to_double(lulesh_double_to_bits(y) +
                                  ((uint64_t)(int64_t)q << 52));

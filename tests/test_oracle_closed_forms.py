@@ -173,6 +173,29 @@ def test_project_cbrt_accuracy(oracle):
     assert L.lo_cbrt_project(0.0) == 0.0
 
 
+def test_project_cbrt_is_correctly_rounded(oracle):
+    """r is the binary64 value nearest to x^(1/3) exactly when x lies strictly
+    between the cubes of the midpoints to r's neighbours; checked in rational
+    arithmetic over the whole exponent range (subnormals included) and at the
+    ends of the reduction interval."""
+    from fractions import Fraction
+    rng = np.random.default_rng(11)
+    L = oracle.lib()
+    xs = list(np.ldexp(1 + rng.uniform(0, 1, 4000), rng.integers(-1074, 1024, 4000)))
+    xs += [np.nextafter(8.0, 0.0), np.nextafter(1.0, 0.0), np.nextafter(1.0, 2.0),
+           2.0, 4.0, 5e-324, np.nextafter(2.2250738585072014e-308, 0.0),
+           2.2250738585072014e-308, 1.7976931348623157e308]
+    for x in xs:
+        x = float(x)
+        if not (0.0 < x < np.inf):
+            continue
+        r = L.lo_cbrt_project(x)
+        below = Fraction(r - np.nextafter(r, 0.0)) / 2
+        above = Fraction(np.nextafter(r, np.inf) - r) / 2
+        assert (Fraction(r) - below) ** 3 < Fraction(x) < (Fraction(r) + above) ** 3, x.hex()
+        assert L.lo_cbrt_project(-x) == -r
+
+
 def test_rand_restatement_matches_c_library(oracle):
     """The region layout draws from rand() after srand(0); the restatement
     reproduces the C library's sequence (glibc here and on the GPU host)."""
