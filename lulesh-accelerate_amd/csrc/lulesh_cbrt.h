@@ -23,7 +23,11 @@
  * applied to the exponent field.
  *
  * The includer defines LULESH_FN (for example `static inline` on the host or
- * `static __device__ inline` in HIP code) before including this header.
+ * `static __device__ inline` in HIP code) before including this header.  It
+ * may also define LULESH_CONST(c), which must return c unchanged: the GPU
+ * build uses it to keep the constants in scalar registers at their use, so
+ * that they do not occupy vector registers across the loops that inline the
+ * root.
  */
 #ifndef LULESH_CBRT_H
 #define LULESH_CBRT_H
@@ -32,6 +36,9 @@
 
 #ifndef LULESH_FN
 #error "define LULESH_FN before including lulesh_cbrt.h"
+#endif
+#ifndef LULESH_CONST
+#define LULESH_CONST(c) (c)
 #endif
 
 LULESH_FN double lulesh_bits_to_double(uint64_t b)
@@ -59,12 +66,17 @@ LULESH_FN double lulesh_cbrt_pos_normal(double x)
     const uint64_t frac = b & 0x000fffffffffffffull;
     const double m = lulesh_bits_to_double(frac | (1023ull << 52));
     const double t = lulesh_bits_to_double(frac | ((uint64_t)(1023 + r) << 52));
-    const double third = 0.3333333333333333;
-    /* 2^(-r/3) */
-    const double c = (r == 0) ? 1.0 : ((r == 1) ? 0.7937005259840998 : 0.6299605249474366);
-    double z = fma(m, fma(m, fma(m, -0.04993701447458462, 0.3177574893441185),
-                          -0.809960645949834),
-                   1.5418870676993346) * c;
+    const double third = LULESH_CONST(0.3333333333333333);
+    double z = fma(m, fma(m, fma(m, LULESH_CONST(-0.04993701447458462),
+                                 LULESH_CONST(0.3177574893441185)),
+                          LULESH_CONST(-0.809960645949834)),
+                   LULESH_CONST(1.5418870676993346));
+    /* times 2^(-r/3) */
+    if (r == 1) {
+        z = z * LULESH_CONST(0.7937005259840998);
+    } else if (r == 2) {
+        z = z * LULESH_CONST(0.6299605249474366);
+    }
     for (int i = 0; i < 2; ++i) {
         const double d = fma(-t, (z * z) * z, 1.0); /* 1 - t z^3 */
         z = fma(z * d, third, z);

@@ -21,6 +21,15 @@
 #endif
 
 #define LULESH_FN static __device__ __forceinline__
+// A double constant placed in scalar registers where it is used: the empty
+// asm cannot be moved out of a loop, so the constant is not hoisted into
+// vector registers that would stay live across the loop.
+static __device__ __forceinline__ double lulesh_sconst(double c)
+{
+    asm volatile("" : "+s"(c));
+    return c;
+}
+#define LULESH_CONST(c) lulesh_sconst(c)
 #include "lulesh_cbrt.h"
 
 namespace lulesh {

@@ -97,12 +97,22 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
     const int je = j0 - 1 + ty;
     const bool col_valid = ie >= 0 && ie < nx && je >= 0 && je < nx;
     const bool owner = tx >= 1 && ty >= 1 && ie <= nx && je <= nx;
-    // which of the four element columns around the owned node exist
-    const bool v_mm = owner && ie >= 1 && je >= 1;
-    const bool v_0m = owner && ie < nx && je >= 1;
-    const bool v_m0 = owner && ie >= 1 && je < nx;
-    const bool v_00 = owner && ie < nx && je < nx;
     const bool hourglass = prm.hgcoef > 0.0;
+    // The corner slots of a thread whose element column lies outside the
+    // domain hold +0.0, so an owned node adds the four corners of a layer
+    // without testing which elements exist.  Adding +0.0 leaves the sum's
+    // bits as skipping the element does: the sum starts from +0.0, and a sum
+    // started from +0.0 is never -0.0.  Only this thread writes these slots;
+    // the first fold's barrier orders the stores before any read.
+    if (!col_valid) {
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                corner[t][c][tid] = 0.0;
+            }
+        }
+    }
 
     // A node plane of the tile is (TX+1) x (TY+1) nodes; thread tid holds the
     // slots tid + r * NT (r < kSlots) of it.  Planes are fetched into
@@ -170,19 +180,19 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
             if (finalize) {
                 double acc = P[t];
                 if (layer) {
-                    if (v_mm) acc += corner[t][2][tid - TX - 1];
-                    if (v_0m) acc += corner[t][3][tid - TX];
-                    if (v_m0) acc += corner[t][1][tid - 1];
-                    if (v_00) acc += corner[t][0][tid];
+                    acc += corner[t][2][tid - TX - 1];
+                    acc += corner[t][3][tid - TX];
+                    acc += corner[t][1][tid - 1];
+                    acc += corner[t][0][tid];
                 }
                 F[t] = acc;
             }
             if (carry) {
                 double acc = 0.0;
-                if (v_mm) acc += corner[t][6][tid - TX - 1];
-                if (v_0m) acc += corner[t][7][tid - TX];
-                if (v_m0) acc += corner[t][5][tid - 1];
-                if (v_00) acc += corner[t][4][tid];
+                acc += corner[t][6][tid - TX - 1];
+                acc += corner[t][7][tid - TX];
+                acc += corner[t][5][tid - 1];
+                acc += corner[t][4][tid];
                 P[t] = acc;
             }
         }
@@ -345,6 +355,10 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
                 }
                 const double determ = field(3) * vv;
                 const double volinv = 1.0 / determ;
+                // the coefficient before hourgam, which holds 32 values live
+                const double volume13 = lulesh_cbrt(determ);
+                const double coefficient =
+                    -prm.hgcoef * 0.01 * field(4) * field(5) / volume13;
                 double hourgam[8][4];
                 {
                     double x[8], y[8], z[8];
@@ -382,9 +396,6 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
                         }
                     }
                 }
-                const double volume13 = lulesh_cbrt(determ);
-                const double coefficient =
-                    -prm.hgcoef * 0.01 * field(4) * field(5) / volume13;
 #pragma unroll
                 for (int t = 0; t < 3; ++t) {
                     double u[8];

@@ -217,7 +217,8 @@ __device__ __forceinline__ void kinematics_q_nodes(const DevFields &f, DevScalar
         raise_error(s, -1);
     }
 
-    // monotonic-Q gradients
+    // monotonic-Q gradients; each root below is of a sum of squares plus
+    // ptiny, so its argument is >= 1e-36 and sqrt_above_tiny is sqrt there
     const double ptiny = 1.e-36;
     const double vol = volo * vnew;
     const double norm = 1.0 / (vol + ptiny);
@@ -235,7 +236,7 @@ __device__ __forceinline__ void kinematics_q_nodes(const DevFields &f, DevScalar
     ax = dyi * dzj - dzi * dyj;
     ay = dzi * dxj - dxi * dzj;
     az = dxi * dyj - dyi * dxj;
-    f.delx_zeta[el] = vol / sqrt(ax * ax + ay * ay + az * az + ptiny);
+    f.delx_zeta[el] = vol / sqrt_above_tiny(ax * ax + ay * ay + az * az + ptiny);
     ax *= norm;
     ay *= norm;
     az *= norm;
@@ -247,7 +248,7 @@ __device__ __forceinline__ void kinematics_q_nodes(const DevFields &f, DevScalar
     ax = dyj * dzk - dzj * dyk;
     ay = dzj * dxk - dxj * dzk;
     az = dxj * dyk - dyj * dxk;
-    f.delx_xi[el] = vol / sqrt(ax * ax + ay * ay + az * az + ptiny);
+    f.delx_xi[el] = vol / sqrt_above_tiny(ax * ax + ay * ay + az * az + ptiny);
     ax *= norm;
     ay *= norm;
     az *= norm;
@@ -259,7 +260,7 @@ __device__ __forceinline__ void kinematics_q_nodes(const DevFields &f, DevScalar
     ax = dyk * dzi - dzk * dyi;
     ay = dzk * dxi - dxk * dzi;
     az = dxk * dyi - dyk * dxi;
-    f.delx_eta[el] = vol / sqrt(ax * ax + ay * ay + az * az + ptiny);
+    f.delx_eta[el] = vol / sqrt_above_tiny(ax * ax + ay * ay + az * az + ptiny);
     ax *= norm;
     ay *= norm;
     az *= norm;

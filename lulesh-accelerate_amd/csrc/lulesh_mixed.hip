@@ -108,11 +108,19 @@ __device__ __forceinline__ void sweep_tile_mixed(const DevFieldsMixed &f, DevSca
     const int je = j0 - 1 + ty;
     const bool col_valid = ie >= 0 && ie < nx && je >= 0 && je < nx;
     const bool owner = tx >= 1 && ty >= 1 && ie <= nx && je <= nx;
-    const bool v_mm = owner && ie >= 1 && je >= 1;
-    const bool v_0m = owner && ie < nx && je >= 1;
-    const bool v_m0 = owner && ie >= 1 && je < nx;
-    const bool v_00 = owner && ie < nx && je < nx;
     const bool hourglass = prm.hgcoef > 0.0;
+    // corner slots of element columns outside the domain hold +0.0, so the
+    // fold adds four corners per layer without testing which elements exist
+    // (adding +0.0 to a sum started from +0.0 changes no bits)
+    if (!col_valid) {
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                corner[t][c][tid] = 0.0f;
+            }
+        }
+    }
     const float hg = float(-prm.hgcoef * 0.01);
 
     constexpr int kSlots = (PN + NT - 1) / NT;
@@ -166,19 +174,19 @@ __device__ __forceinline__ void sweep_tile_mixed(const DevFieldsMixed &f, DevSca
             if (finalize) {
                 float acc = P[t];
                 if (layer) {
-                    if (v_mm) acc += corner[t][2][tid - TX - 1];
-                    if (v_0m) acc += corner[t][3][tid - TX];
-                    if (v_m0) acc += corner[t][1][tid - 1];
-                    if (v_00) acc += corner[t][0][tid];
+                    acc += corner[t][2][tid - TX - 1];
+                    acc += corner[t][3][tid - TX];
+                    acc += corner[t][1][tid - 1];
+                    acc += corner[t][0][tid];
                 }
                 F[t] = acc;
             }
             if (carry) {
                 float acc = 0.0f;
-                if (v_mm) acc += corner[t][6][tid - TX - 1];
-                if (v_0m) acc += corner[t][7][tid - TX];
-                if (v_m0) acc += corner[t][5][tid - 1];
-                if (v_00) acc += corner[t][4][tid];
+                acc += corner[t][6][tid - TX - 1];
+                acc += corner[t][7][tid - TX];
+                acc += corner[t][5][tid - 1];
+                acc += corner[t][4][tid];
                 P[t] = acc;
             }
         }
