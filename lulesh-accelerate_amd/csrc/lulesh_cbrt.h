@@ -55,7 +55,8 @@ LULESH_FN uint64_t lulesh_double_to_bits(double d)
     return c.u;
 }
 
-/* Cube root of a positive normal binary64 value. */
+/* Cube root of a positive normal binary64 value (for other inputs it returns
+ * some finite value). */
 LULESH_FN double lulesh_cbrt_pos_normal(double x)
 {
     const uint64_t b = lulesh_double_to_bits(x);
@@ -72,10 +73,10 @@ LULESH_FN double lulesh_cbrt_pos_normal(double x)
                           LULESH_CONST(-0.809960645949834)),
                    LULESH_CONST(1.5418870676993346));
     /* times 2^(-r/3) */
-    if (r == 1) {
-        z = z * LULESH_CONST(0.7937005259840998);
-    } else if (r == 2) {
-        z = z * LULESH_CONST(0.6299605249474366);
+    {
+        const double zc = z * ((r == 1) ? LULESH_CONST(0.7937005259840998)
+                                        : LULESH_CONST(0.6299605249474366));
+        z = (r == 0) ? z : zc;
     }
     for (int i = 0; i < 2; ++i) {
         const double d = fma(-t, (z * z) * z, 1.0); /* 1 - t z^3 */
@@ -93,26 +94,21 @@ LULESH_FN double lulesh_cbrt_pos_normal(double x)
                                  ((uint64_t)(int64_t)q << 52));
 }
 
+/* Every input takes the same steps (selects, no branches): the root of |x|,
+ * subnormals scaled by 2^54 into the normal range first and the root by
+ * 2^-18 after (both exact), then x's sign; zeros, infinities and NaNs return
+ * x + x, whatever the core computed for them. */
 LULESH_FN double lulesh_cbrt(double x)
 {
     const uint64_t b = lulesh_double_to_bits(x);
     const uint64_t ex = (b >> 52) & 0x7ffu;
-    if (ex != 0 && ex != 0x7ffu) {
-        if (x > 0.0) {
-            return lulesh_cbrt_pos_normal(x);
-        }
-        return -lulesh_cbrt_pos_normal(-x);
-    }
-    if (ex == 0x7ffu || x == 0.0) {
-        return x + x; /* inf, nan and signed zero pass through */
-    }
-    /* subnormal: scale by 2^54 (exact), take the root, scale by 2^-18 */
-    {
-        const double s = x * 18014398509481984.0;
-        const double r = (s > 0.0) ? lulesh_cbrt_pos_normal(s)
-                                   : -lulesh_cbrt_pos_normal(-s);
-        return r * 3.814697265625e-06;
-    }
+    const uint64_t sign = b & 0x8000000000000000ull;
+    const double ax = lulesh_bits_to_double(b ^ sign);
+    const double s = (ex == 0) ? ax * 18014398509481984.0 : ax;
+    double r = lulesh_cbrt_pos_normal(s);
+    r = (ex == 0) ? r * 3.814697265625e-06 : r;
+    r = lulesh_bits_to_double(lulesh_double_to_bits(r) | sign);
+    return (ex == 0x7ffu || x == 0.0) ? x + x : r;
 }
 
 #endif /* LULESH_CBRT_H */
