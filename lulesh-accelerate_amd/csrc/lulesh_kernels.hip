@@ -187,12 +187,22 @@ __device__ __forceinline__ void kinematics_q_nodes(const DevFields &f, DevScalar
         zv[c] = pzd[nodes[c]];
     }
     const double volo = f.volo[el];
+    const double v_old = f.v[el];  // read before the stores below may alias it
+    // The empty asm takes every node value as an operand, so all the loads are
+    // issued before any arithmetic and the element waits for memory once.
+    // Without it the compiler issues the velocity loads after the volume
+    // arithmetic, and each element waits for two round trips.
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        asm volatile("" : "+v"(x[c]), "+v"(y[c]), "+v"(z[c]), "+v"(xv[c]), "+v"(yv[c]),
+                     "+v"(zv[c]));
+    }
 
     // kinematics
     const double volume = elem_volume(x, y, z);
     const double vnew = volume / volo;
     f.vnew[el] = vnew;
-    f.delv[el] = vnew - f.v[el];
+    f.delv[el] = vnew - v_old;
     f.arealg[el] = characteristic_length(x, y, z, volume);
     {
         const double dt2 = 0.5 * dt;
