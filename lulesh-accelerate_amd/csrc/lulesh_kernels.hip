@@ -394,6 +394,9 @@ struct QInputs {
     double du[3];
     double rho;
     double vdov, vnew, volo;
+    // the element's fields that the EOS and the time constraints read; they
+    // are loaded with the limiter's inputs, before the kernel stores anything
+    double q_old, v_old, e_old, delv, p_old, arealg;
 };
 
 // the rest of the inputs once the limiter values are known
@@ -406,14 +409,19 @@ __device__ __forceinline__ void q_inputs(const DevFields &f, const PhysParams &p
     const int nx = f.nx;
     const int sy = nx, sz = nx * nx;
     const double dvxi = f.delv_xi[el], dveta = f.delv_eta[el], dvzeta = f.delv_zeta[el];
-    const double lm = prm.monoq_limiter_mult, ms = prm.monoq_max_slope;
-    in.phi[0] = mono_phi(dvxi, (i == 0) ? dvxi : f.delv_xi[el - 1],
-                         (i == nx - 1) ? 0.0 : f.delv_xi[el + 1], lm, ms);
-    in.phi[1] = mono_phi(dveta, (j == 0) ? dveta : f.delv_eta[el - sy],
-                         (j == nx - 1) ? 0.0 : f.delv_eta[el + sy], lm, ms);
-    in.phi[2] = mono_phi(dvzeta, (k == 0) ? dvzeta : f.delv_zeta[el - sz],
-                         (k == nx - 1) ? 0.0 : f.delv_zeta[el + sz], lm, ms);
+    // every neighbour is loaded, from the element itself across a boundary
+    // face (the value is then replaced), so that no load waits on a branch
+    const double xm = f.delv_xi[(i == 0) ? el : el - 1];
+    const double xp = f.delv_xi[(i == nx - 1) ? el : el + 1];
+    const double ym = f.delv_eta[(j == 0) ? el : el - sy];
+    const double yp = f.delv_eta[(j == nx - 1) ? el : el + sy];
+    const double zm = f.delv_zeta[(k == 0) ? el : el - sz];
+    const double zp = f.delv_zeta[(k == nx - 1) ? el : el + sz];
     q_inputs_rest(f, el, dvxi, dveta, dvzeta, in);
+    const double lm = prm.monoq_limiter_mult, ms = prm.monoq_max_slope;
+    in.phi[0] = mono_phi(dvxi, (i == 0) ? dvxi : xm, (i == nx - 1) ? 0.0 : xp, lm, ms);
+    in.phi[1] = mono_phi(dveta, (j == 0) ? dveta : ym, (j == nx - 1) ? 0.0 : yp, lm, ms);
+    in.phi[2] = mono_phi(dvzeta, (k == 0) ? dvzeta : zm, (k == nx - 1) ? 0.0 : zp, lm, ms);
 }
 
 // the neighbour value across one face of an unstructured element: own value
@@ -455,6 +463,14 @@ __device__ __forceinline__ void q_inputs_rest(const DevFields &f, int el, double
     in.vdov = f.vdov[el];
     in.vnew = f.vnew[el];
     in.volo = f.volo[el];
+    in.q_old = f.q[el];
+    in.v_old = f.v[el];
+    in.e_old = f.e[el];
+    in.delv = f.delv[el];
+    in.p_old = f.p[el];
+    // arealg is read only by a moving element, delx and the mass only by a
+    // compressing or resting one (a quiescent element reads neither arealg)
+    in.arealg = (in.vdov != 0.) ? f.arealg[el] : 0.0;
     in.du[0] = in.du[1] = in.du[2] = 0.0;
     in.rho = 0.0;
     if (!(in.vdov > 0.)) {
@@ -494,16 +510,17 @@ __device__ __forceinline__ void monotonic_q(const PhysParams &prm, const QInputs
 // Courant / hydro candidates.
 __device__ __forceinline__ void eos_dt_elem(const DevFields &f, DevScalars *s,
                                             const PhysParams &prm, int el, int rep,
-                                            double vdov, double vnew, double qlin,
-                                            double qquad, unsigned long long &cand_c,
+                                            const QInputs &in, double qlin, double qquad,
+                                            unsigned long long &cand_c,
                                             unsigned long long &cand_h)
 {
+    const double vdov = in.vdov, vnew = in.vnew;
     {
         f.ql[el] = qlin;
         f.qq[el] = qquad;
 
         // qstop is tested on the q of the previous cycle (:2368-2381)
-        const double q_old = f.q[el];
+        const double q_old = in.q_old;
         if (q_old > prm.qstop) {
             raise_error(s, -2);
         }
@@ -516,7 +533,7 @@ __device__ __forceinline__ void eos_dt_elem(const DevFields &f, DevScalars *s,
         if (prm.eosvmax != 0.) {
             if (vnewc > prm.eosvmax) vnewc = prm.eosvmax;
         }
-        double vc = f.v[el];
+        double vc = in.v_old;
         if (prm.eosvmin != 0.) {
             if (vc < prm.eosvmin) vc = prm.eosvmin;
         }
@@ -539,7 +556,7 @@ __device__ __forceinline__ void eos_dt_elem(const DevFields &f, DevScalars *s,
         ec.unit_rho0 = prm.refdens == 1.0;
         ec.ss_tiny = prm.ss_tiny;
         ec.ss_floor = prm.ss_floor;
-        const double e_old = f.e[el], delvc = f.delv[el], p_old = f.p[el];
+        const double e_old = in.e_old, delvc = in.delv, p_old = in.p_old;
         EosOut o;
         o.p = o.e = o.q = o.ss = 0.0;
         // the region's repetitions redo the same evaluation, as the
@@ -581,7 +598,7 @@ __device__ __forceinline__ void eos_dt_elem(const DevFields &f, DevScalars *s,
 
         // Courant and hydro constraints (:2799-2933)
         if (vdov != 0.) {
-            const double arealg = f.arealg[el];
+            const double arealg = in.arealg;
             const double qqc2 = 64.0 * prm.qqc * prm.qqc;
             double dtf = o.ss * o.ss;
             if (vdov < 0.) {
@@ -629,7 +646,7 @@ __device__ __forceinline__ void q_eos_dt_block(const DevFields &f, DevScalars *s
         q_inputs(f, prm, el, i, j, k, in);
         double qlin, qquad;
         monotonic_q(prm, in, qlin, qquad);
-        eos_dt_elem(f, s, prm, el, rep, in.vdov, in.vnew, qlin, qquad, cand_c, cand_h);
+        eos_dt_elem(f, s, prm, el, rep, in, qlin, qquad, cand_c, cand_h);
     }
     fold_minima(s, cand_c, cand_h, lds_c, lds_h);
 }
@@ -822,7 +839,7 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_learned(DevFields f, DevScalar
     }
     unsigned long long cand_c = as_bits(kBig), cand_h = as_bits(kBig);
     if (valid) {
-        eos_dt_elem(f, s, prm, el, rep, in.vdov, in.vnew, qlin, qquad, cand_c, cand_h);
+        eos_dt_elem(f, s, prm, el, rep, in, qlin, qquad, cand_c, cand_h);
     }
     fold_minima(s, cand_c, cand_h, lds_c, lds_h);
 }
@@ -870,7 +887,7 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_u(DevFields f, DevMeshU m, Dev
         q_inputs_u(f, m, prm, el, in);
         double qlin, qquad;
         monotonic_q(prm, in, qlin, qquad);
-        eos_dt_elem(f, s, prm, el, rep, in.vdov, in.vnew, qlin, qquad, cand_c, cand_h);
+        eos_dt_elem(f, s, prm, el, rep, in, qlin, qquad, cand_c, cand_h);
     }
     fold_minima(s, cand_c, cand_h, lds_c, lds_h);
 }
