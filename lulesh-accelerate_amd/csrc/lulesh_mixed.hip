@@ -604,11 +604,19 @@ __global__ __launch_bounds__(256) void k_kinematics_q_mixed(DevFieldsMixed f,
         ZV[c] = f.zd[par][nodes[c]];
     }
     const double volo = f.volo[el];
+    const double v_old = f.v[el];  // read before the stores below may alias it
+    // all loads issued before any arithmetic: one wait for memory (as in
+    // k_kinematics_q)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        asm volatile("" : "+v"(X[c]), "+v"(Y[c]), "+v"(Z[c]), "+v"(XV[c]), "+v"(YV[c]),
+                     "+v"(ZV[c]));
+    }
     // float64: element volume, relative volume and its change
     const double volume = elem_volume(X, Y, Z);
     const double vnew = volume / volo;
     f.vnew[el] = vnew;
-    f.delv[el] = float(vnew - f.v[el]);
+    f.delv[el] = float(vnew - v_old);
     if (vnew <= 0.0) {
         raise_error(s, -1);
     }
@@ -724,16 +732,29 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_mixed(DevFieldsMixed f, DevSca
         const int sy = nx, sz = nx * nx;
 
         const float dvxi = f.delv_xi[el], dveta = f.delv_eta[el], dvzeta = f.delv_zeta[el];
-        const float lm = float(prm.monoq_limiter_mult), ms = float(prm.monoq_max_slope);
-        const float phixi = mono_phi(dvxi, (i == 0) ? dvxi : f.delv_xi[el - 1],
-                                     (i == nx - 1) ? 0.f : f.delv_xi[el + 1], lm, ms);
-        const float phieta = mono_phi(dveta, (j == 0) ? dveta : f.delv_eta[el - sy],
-                                      (j == nx - 1) ? 0.f : f.delv_eta[el + sy], lm, ms);
-        const float phizeta = mono_phi(dvzeta, (k == 0) ? dvzeta : f.delv_zeta[el - sz],
-                                       (k == nx - 1) ? 0.f : f.delv_zeta[el + sz], lm, ms);
+        // the element's inputs are loaded before its first store (as in
+        // k_q_eos_dt); neighbours across a boundary face come from the
+        // element itself and are then replaced
+        const float xm = f.delv_xi[(i == 0) ? el : el - 1];
+        const float xp = f.delv_xi[(i == nx - 1) ? el : el + 1];
+        const float ym = f.delv_eta[(j == 0) ? el : el - sy];
+        const float yp = f.delv_eta[(j == nx - 1) ? el : el + sy];
+        const float zm = f.delv_zeta[(k == 0) ? el : el - sz];
+        const float zp = f.delv_zeta[(k == nx - 1) ? el : el + sz];
         const float vdov = f.vdov[el];
         const double vnew = f.vnew[el];
         const double volo = f.volo[el];
+        const float q_old = f.q[el];
+        const double v_old = f.v[el];
+        const double e_old = f.e[el];
+        const float delvc = f.delv[el], p_old = f.p[el];
+        const float arealg = (vdov != 0.f) ? f.arealg[el] : 0.f;
+        const float lm = float(prm.monoq_limiter_mult), ms = float(prm.monoq_max_slope);
+        const float phixi = mono_phi(dvxi, (i == 0) ? dvxi : xm, (i == nx - 1) ? 0.f : xp, lm, ms);
+        const float phieta =
+            mono_phi(dveta, (j == 0) ? dveta : ym, (j == nx - 1) ? 0.f : yp, lm, ms);
+        const float phizeta =
+            mono_phi(dvzeta, (k == 0) ? dvzeta : zm, (k == nx - 1) ? 0.f : zp, lm, ms);
         float qlin, qquad;
         if (vdov > 0.f) {
             qlin = 0.f;
@@ -755,7 +776,6 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_mixed(DevFieldsMixed f, DevSca
         f.ql[el] = qlin;
         f.qq[el] = qquad;
 
-        const float q_old = f.q[el];
         if (double(q_old) > prm.qstop) {
             raise_error(s, -2);
         }
@@ -766,7 +786,7 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_mixed(DevFieldsMixed f, DevSca
         if (prm.eosvmax != 0.) {
             if (vnewc > prm.eosvmax) vnewc = prm.eosvmax;
         }
-        double vc = f.v[el];
+        double vc = v_old;
         if (prm.eosvmin != 0.) {
             if (vc < prm.eosvmin) vc = prm.eosvmin;
         }
@@ -789,8 +809,6 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_mixed(DevFieldsMixed f, DevSca
         ec.unit_rho0 = prm.refdens == 1.0;
         ec.ss_tiny = float(prm.ss_tiny);
         ec.ss_floor = float(prm.ss_floor);
-        const double e_old = f.e[el];
-        const float delvc = f.delv[el], p_old = f.p[el];
         EosOutMixed o;
         o.e = 0.0;
         o.p = o.q = o.ss = 0.f;
@@ -818,7 +836,6 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_mixed(DevFieldsMixed f, DevSca
         }
 
         if (vdov != 0.f) {
-            const float arealg = f.arealg[el];
             const float qqc2 = float(64.0 * prm.qqc * prm.qqc);
             float dtf = o.ss * o.ss;
             if (vdov < 0.f) {
