@@ -164,7 +164,9 @@ __global__ void k_time_increment_batch(const BatchMember *__restrict__ members,
 // ---------------------------------------------------------------------------
 namespace {
 
-// the element's eight nodes are given (structured or from a node list)
+// the element's eight nodes are given (structured or from a node list);
+// OneTrip: all node loads before any arithmetic (see below)
+template <bool OneTrip>
 __device__ __forceinline__ void kinematics_q_nodes(const DevFields &f, DevScalars *s,
                                                    int el, const int (&nodes)[8])
 {
@@ -191,11 +193,14 @@ __device__ __forceinline__ void kinematics_q_nodes(const DevFields &f, DevScalar
     // The empty asm takes every node value as an operand, so all the loads are
     // issued before any arithmetic and the element waits for memory once.
     // Without it the compiler issues the velocity loads after the volume
-    // arithmetic, and each element waits for two round trips.
+    // arithmetic, and each element waits for two round trips.  The list form
+    // (whose node indices are loads themselves) runs faster without it.
+    if constexpr (OneTrip) {
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        asm volatile("" : "+v"(x[c]), "+v"(y[c]), "+v"(z[c]), "+v"(xv[c]), "+v"(yv[c]),
-                     "+v"(zv[c]));
+        for (int c = 0; c < 8; ++c) {
+            asm volatile("" : "+v"(x[c]), "+v"(y[c]), "+v"(z[c]), "+v"(xv[c]), "+v"(yv[c]),
+                         "+v"(zv[c]));
+        }
     }
 
     // kinematics
@@ -289,7 +294,7 @@ __device__ __forceinline__ void kinematics_q(const DevFields &f, DevScalars *s, 
     const int pl = en * en;
     const int nodes[8] = {n0,      n0 + 1,      n0 + en + 1,      n0 + en,
                           n0 + pl, n0 + pl + 1, n0 + pl + en + 1, n0 + pl + en};
-    kinematics_q_nodes(f, s, el, nodes);
+    kinematics_q_nodes<true>(f, s, el, nodes);
 }
 
 }  // namespace
@@ -862,7 +867,7 @@ __global__ __launch_bounds__(256) void k_kinematics_q_u(DevFields f, DevMeshU m,
         for (int c = 0; c < 8; ++c) {
             nodes[c] = m.nodelist[c * ne + el];
         }
-        kinematics_q_nodes(f, s, el, nodes);
+        kinematics_q_nodes<false>(f, s, el, nodes);
     }
 }
 
