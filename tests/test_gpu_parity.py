@@ -64,6 +64,16 @@ def test_device_math_matches_host(lulesh_gpu, oracle):
     L = oracle.lib()
     host_cbrt = np.array([L.lo_cbrt_project(float(v)) for v in a[:20000]])
     assert bits_equal(lulesh_gpu.device_math("cbrt", a[:20000]), host_cbrt)
+    # the cube root over every class of input: random bit patterns (both
+    # signs, the whole exponent range, subnormals), zeros and infinities
+    bits = rng.integers(0, 2**63, 20000, dtype=np.uint64, endpoint=False)
+    bits[::2] |= np.uint64(1) << np.uint64(63)
+    c = bits.view(np.float64)
+    c = np.concatenate([c[~np.isnan(c)], [0.0, -0.0, np.inf, -np.inf, 5e-324, -5e-324,
+                                          2.2250738585072014e-308, 1.7976931348623157e308]])
+    host_c = np.array([L.lo_cbrt_project(float(v)) for v in c])
+    assert bits_equal(lulesh_gpu.device_math("cbrt", c), host_c)
+    assert np.isnan(lulesh_gpu.device_math("cbrt", np.array([np.nan]))[0])
     assert bits_equal(lulesh_gpu.device_math("sqrt", a), np.sqrt(a))
     assert bits_equal(lulesh_gpu.device_math("div", a, b), a / b)
 
