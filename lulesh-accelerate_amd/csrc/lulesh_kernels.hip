@@ -431,13 +431,13 @@ __device__ __forceinline__ void q_inputs(const DevFields &f, const PhysParams &p
 
 // the neighbour value across one face of an unstructured element: own value
 // on a symmetry face, 0 on a free face, else the neighbour's
-// (reference/C/LULESH-OMP.cc:2201-2287)
-__device__ __forceinline__ double face_value(const double *a, int el, int nbr, int bc,
-                                             int symm, int freeb)
+// (reference/C/LULESH-OMP.cc:2201-2287).  The neighbour's value is loaded in
+// every case (the mesh builder checks that every neighbour index is an
+// element), so that no load waits on the boundary bits.
+__device__ __forceinline__ double face_value(double own, double nbr_value, int bc, int symm,
+                                             int freeb)
 {
-    if (bc & symm) return a[el];
-    if (bc & freeb) return 0.0;
-    return a[nbr];
+    return (bc & symm) ? own : ((bc & freeb) ? 0.0 : nbr_value);
 }
 
 __device__ __forceinline__ void q_inputs_u(const DevFields &f, const DevMeshU &m,
@@ -446,20 +446,17 @@ __device__ __forceinline__ void q_inputs_u(const DevFields &f, const DevMeshU &m
     const int ne = f.num_elem;
     const int bc = m.elem_bc[el];
     const double dvxi = f.delv_xi[el], dveta = f.delv_eta[el], dvzeta = f.delv_zeta[el];
-    const double lm = prm.monoq_limiter_mult, ms = prm.monoq_max_slope;
-    in.phi[0] = mono_phi(dvxi,
-                         face_value(f.delv_xi, el, m.nbr[0 * ne + el], bc, kBcXiMSymm, kBcXiMFree),
-                         face_value(f.delv_xi, el, m.nbr[1 * ne + el], bc, kBcXiPSymm, kBcXiPFree),
-                         lm, ms);
-    in.phi[1] = mono_phi(dveta,
-                         face_value(f.delv_eta, el, m.nbr[2 * ne + el], bc, kBcEtaMSymm, kBcEtaMFree),
-                         face_value(f.delv_eta, el, m.nbr[3 * ne + el], bc, kBcEtaPSymm, kBcEtaPFree),
-                         lm, ms);
-    in.phi[2] = mono_phi(dvzeta,
-                         face_value(f.delv_zeta, el, m.nbr[4 * ne + el], bc, kBcZetaMSymm, kBcZetaMFree),
-                         face_value(f.delv_zeta, el, m.nbr[5 * ne + el], bc, kBcZetaPSymm, kBcZetaPFree),
-                         lm, ms);
+    const double xm = f.delv_xi[m.nbr[0 * ne + el]], xp = f.delv_xi[m.nbr[1 * ne + el]];
+    const double ym = f.delv_eta[m.nbr[2 * ne + el]], yp = f.delv_eta[m.nbr[3 * ne + el]];
+    const double zm = f.delv_zeta[m.nbr[4 * ne + el]], zp = f.delv_zeta[m.nbr[5 * ne + el]];
     q_inputs_rest(f, el, dvxi, dveta, dvzeta, in);
+    const double lm = prm.monoq_limiter_mult, ms = prm.monoq_max_slope;
+    in.phi[0] = mono_phi(dvxi, face_value(dvxi, xm, bc, kBcXiMSymm, kBcXiMFree),
+                         face_value(dvxi, xp, bc, kBcXiPSymm, kBcXiPFree), lm, ms);
+    in.phi[1] = mono_phi(dveta, face_value(dveta, ym, bc, kBcEtaMSymm, kBcEtaMFree),
+                         face_value(dveta, yp, bc, kBcEtaPSymm, kBcEtaPFree), lm, ms);
+    in.phi[2] = mono_phi(dvzeta, face_value(dvzeta, zm, bc, kBcZetaMSymm, kBcZetaMFree),
+                         face_value(dvzeta, zp, bc, kBcZetaPSymm, kBcZetaPFree), lm, ms);
 }
 
 __device__ __forceinline__ void q_inputs_rest(const DevFields &f, int el, double dvxi,
