@@ -638,22 +638,48 @@ __global__ __launch_bounds__(256) void k_nodes_u(DevFields f, DevMeshU m, DevSca
     const int par = s->cur;
     const double dt = s->deltatime;
     const int b = m.corner_start[n], e = m.corner_start[n + 1];
-    double sx = 0.0, sy = 0.0, sz = 0.0;
-    for (int i = b; i < e; ++i) {
-        const int c = m.corner_list[i];
-        sx += m.fs[0][c];
-        sy += m.fs[1][c];
-        sz += m.fs[2][c];
+    const int cnt = e - b;
+    // The first eight corners (all of them on a hexahedral mesh where at most
+    // eight elements meet) are loaded together, without a branch per corner:
+    // an absent one reads corner-list entry 0 and adds +0.0 instead, which
+    // leaves the sum's bits as skipping it does (a sum started from +0.0 is
+    // never -0.0).  Further corners follow in list order.
+    int ci[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        ci[c] = m.corner_list[(c < cnt) ? b + c : 0];
     }
-    double fx = sx, fy = sy, fz = sz;
-    if (prm.hgcoef > 0.0) {
-        double hx = 0.0, hy = 0.0, hz = 0.0;
-        for (int i = b; i < e; ++i) {
-            const int c = m.corner_list[i];
-            hx += m.fh[0][c];
-            hy += m.fh[1][c];
-            hz += m.fh[2][c];
+    auto gather = [&](const double *const *a, double &gx, double &gy, double &gz) {
+        double v[3][8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                v[t][c] = a[t][ci[c]];
+            }
         }
+        double ax = 0.0, ay = 0.0, az = 0.0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            ax += (c < cnt) ? v[0][c] : 0.0;
+            ay += (c < cnt) ? v[1][c] : 0.0;
+            az += (c < cnt) ? v[2][c] : 0.0;
+        }
+        for (int i = b + 8; i < e; ++i) {
+            const int c = m.corner_list[i];
+            ax += a[0][c];
+            ay += a[1][c];
+            az += a[2][c];
+        }
+        gx = ax;
+        gy = ay;
+        gz = az;
+    };
+    double fx, fy, fz;
+    gather(m.fs, fx, fy, fz);
+    if (prm.hgcoef > 0.0) {
+        double hx, hy, hz;
+        gather(m.fh, hx, hy, hz);
         fx += hx;
         fy += hy;
         fz += hz;
