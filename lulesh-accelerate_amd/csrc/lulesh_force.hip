@@ -541,8 +541,19 @@ __global__ __launch_bounds__(256) void k_corner_forces_u(DevFields f, DevMeshU m
         y[c] = f.y[par][nodes[c]];
         z[c] = f.z[par][nodes[c]];
     }
+    // everything the element reads is loaded before its first store, which
+    // the compiler must assume may alias it: one wait for memory, not three
+    const double p_el = f.p[el], q_el = f.q[el];
+    const double vv = f.v[el], volo = f.volo[el], ss = f.ss[el], emass = f.elem_mass[el];
+    double vel[3][8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        vel[0][c] = f.xd[par][nodes[c]];
+        vel[1][c] = f.yd[par][nodes[c]];
+        vel[2][c] = f.zd[par][nodes[c]];
+    }
     {
-        const double sig = -f.p[el] - f.q[el];
+        const double sig = -p_el - q_el;
         const double det = jacobian_volume(x, y, z);
         if (det <= 0.0) {
             atomicCAS(&s->error, 0, -1);
@@ -560,11 +571,10 @@ __global__ __launch_bounds__(256) void k_corner_forces_u(DevFields f, DevMeshU m
     if (!(prm.hgcoef > 0.0)) {
         return;
     }
-    const double vv = f.v[el];
     if (vv <= 0.0) {
         atomicCAS(&s->error, 0, -1);
     }
-    const double determ = f.volo[el] * vv;
+    const double determ = volo * vv;
     const double volinv = 1.0 / determ;
     double hourgam[8][4];
     {
@@ -597,15 +607,10 @@ __global__ __launch_bounds__(256) void k_corner_forces_u(DevFields f, DevMeshU m
         }
     }
     const double volume13 = lulesh_cbrt(determ);
-    const double coefficient = -prm.hgcoef * 0.01 * f.ss[el] * f.elem_mass[el] / volume13;
-    const double *vel[3] = {f.xd[par], f.yd[par], f.zd[par]};
+    const double coefficient = -prm.hgcoef * 0.01 * ss * emass / volume13;
 #pragma unroll
     for (int t = 0; t < 3; ++t) {
-        double u[8];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            u[c] = vel[t][nodes[c]];
-        }
+        const double *u = vel[t];
         double h[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
