@@ -175,6 +175,13 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
             advance(0);
         }
         __syncthreads();
+        // Wave priority rises as a barrier approaches: 2 in a fold, 1 in the
+        // nodal update, the stress arithmetic and the hourglass projection,
+        // 0 in the hourglass's volume derivatives.  The two blocks on a CU
+        // share every SIMD and each waits at its barriers for its slowest
+        // wave, so a wave near its barrier issues ahead of the other block's
+        // wave far from one (150^3: 301 -> 295 us).
+        __builtin_amdgcn_s_setprio(2);
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
             if (finalize) {
@@ -200,6 +207,11 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
             advance(1);
         }
         __syncthreads();
+        if (last) {
+            __builtin_amdgcn_s_setprio(1);  // nodal update, then the next stress
+        } else {
+            __builtin_amdgcn_s_setprio(0);  // the hourglass's volume derivatives
+        }
     };
 
     const int base = ty * PX + tx;
@@ -396,6 +408,7 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
                         }
                     }
                 }
+                __builtin_amdgcn_s_setprio(1);  // the projection, before the fold
 #pragma unroll
                 for (int t = 0; t < 3; ++t) {
                     double u[8];
