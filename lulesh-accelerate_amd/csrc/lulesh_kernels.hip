@@ -579,11 +579,13 @@ __device__ __forceinline__ void eos_dt_elem(const DevFields &f, DevScalars *s,
                 asm volatile("" ::"v"(o.p), "v"(o.e), "v"(o.q), "v"(o.ss));
             }
         };
+        __builtin_amdgcn_s_setprio(0);  // the repetitions yield to waves still loading
         if (ec.unit_rho0) {
             repeat(std::true_type{});
         } else {
             repeat(std::false_type{});
         }
+        __builtin_amdgcn_s_setprio(1);  // stores and the block's reduction
         f.p[el] = o.p;
         f.e[el] = o.e;
         f.q[el] = o.q;
@@ -671,6 +673,7 @@ __global__ __launch_bounds__(256) void k_q_eos_dt(DevFields f, DevScalars *s,
         return;
     }
     const int chunk = eos_chunk(int(blockIdx.x), int(gridDim.x), bands);
+    __builtin_amdgcn_s_setprio(1);  // the element's loads first
     q_eos_dt_block(f, s, prm, chunk * int(blockDim.x) + int(threadIdx.x), lds_c, lds_h);
 }
 
