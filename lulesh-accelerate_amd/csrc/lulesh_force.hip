@@ -175,12 +175,12 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
             advance(0);
         }
         __syncthreads();
-        // Wave priority rises as a barrier approaches: 2 in a fold, 1 in the
-        // nodal update, the stress arithmetic and the hourglass projection,
-        // 0 in the hourglass's volume derivatives.  The two blocks on a CU
-        // share every SIMD and each waits at its barriers for its slowest
-        // wave, so a wave near its barrier issues ahead of the other block's
-        // wave far from one (150^3: 301 -> 295 us).
+        // Wave priority rises as a barrier approaches: 2 in a fold and the
+        // nodal update, 1 in the stress arithmetic and the hourglass
+        // projection, 0 in the hourglass's volume derivatives.  The two
+        // blocks on a CU share every SIMD and each waits at its barriers for
+        // its slowest wave, so a wave near its barrier issues ahead of the
+        // other block's wave far from one (150^3: 301 -> 291 us).
         __builtin_amdgcn_s_setprio(2);
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
@@ -208,7 +208,7 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
         }
         __syncthreads();
         if (last) {
-            __builtin_amdgcn_s_setprio(1);  // nodal update, then the next stress
+            __builtin_amdgcn_s_setprio(2);  // nodal update
         } else {
             __builtin_amdgcn_s_setprio(0);  // the hourglass's volume derivatives
         }
@@ -338,6 +338,7 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
             }
         };
 
+        __builtin_amdgcn_s_setprio(1);
         // ---- stress: f = -sigma * node normal (:864-903) ----
         if (compute) {
             double x[8], y[8], z[8], pf[3][8];
