@@ -167,8 +167,12 @@ __device__ __forceinline__ void sweep_tile_mixed(const DevFieldsMixed &f, DevSca
             }
         }
     };
+    // wave priority rises as a barrier approaches, as in the float64 sweep:
+    // 2 in the folds, the nodal update and the plane store, 1 in the stress
+    // arithmetic and the hourglass projection, 0 in the volume derivatives
     auto fold = [&](bool finalize, bool carry, bool layer, float *F, float *P) {
         __syncthreads();
+        __builtin_amdgcn_s_setprio(2);
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
             if (finalize) {
@@ -236,6 +240,7 @@ __device__ __forceinline__ void sweep_tile_mixed(const DevFieldsMixed &f, DevSca
         if (layer && k + 1 < nx && k + 1 < kc1) {
             fetch_plane(k + 2);
         }
+        __builtin_amdgcn_s_setprio(1);
         const int lo = k & 1;
         const int hi = lo ^ 1;
         const bool finalize = owner && k >= kc0;
@@ -293,6 +298,7 @@ __device__ __forceinline__ void sweep_tile_mixed(const DevFieldsMixed &f, DevSca
 
         // ---- hourglass control ----
         if (hourglass) {
+            __builtin_amdgcn_s_setprio(0);
             if (compute) {
                 const double vv = ev;
                 if (vv <= 0.0) {
@@ -342,6 +348,7 @@ __device__ __forceinline__ void sweep_tile_mixed(const DevFieldsMixed &f, DevSca
                         }
                     }
                 }
+                __builtin_amdgcn_s_setprio(1);
                 const float coefficient =
                     hg * ess * float(emass) * frcp(fcbrt(determ));
 #pragma unroll
@@ -374,6 +381,7 @@ __device__ __forceinline__ void sweep_tile_mixed(const DevFieldsMixed &f, DevSca
         }
 
         // ---- acceleration, symmetry planes, velocity, position: float64 ----
+        __builtin_amdgcn_s_setprio(2);
         if (finalize) {
             double fx = double(F[0]), fy = double(F[1]), fz = double(F[2]);
             if (hourglass) {
