@@ -201,6 +201,7 @@ __device__ __forceinline__ void kinematics_q_nodes(const DevFields &f, DevScalar
             asm volatile("" : "+v"(x[c]), "+v"(y[c]), "+v"(z[c]), "+v"(xv[c]), "+v"(yv[c]),
                          "+v"(zv[c]));
         }
+        __builtin_amdgcn_s_setprio(1);  // a wave with its data finishes first
     }
 
     // kinematics
