@@ -620,6 +620,7 @@ __global__ __launch_bounds__(256) void k_kinematics_q_mixed(DevFieldsMixed f,
         asm volatile("" : "+v"(X[c]), "+v"(Y[c]), "+v"(Z[c]), "+v"(XV[c]), "+v"(YV[c]),
                      "+v"(ZV[c]));
     }
+    __builtin_amdgcn_s_setprio(1);  // a wave with its data finishes first
     // float64: element volume, relative volume and its change
     const double volume = elem_volume(X, Y, Z);
     const double vnew = volume / volo;
@@ -730,6 +731,7 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_mixed(DevFieldsMixed f, DevSca
     }
     const int slot = eos_chunk(int(blockIdx.x), int(gridDim.x), bands) * int(blockDim.x) +
                      int(threadIdx.x);
+    __builtin_amdgcn_s_setprio(1);
     unsigned long long cand_c = as_bits(kBig), cand_h = as_bits(kBig);
     if (slot < f.num_elem) {
         const int el = f.eos_order[slot];
@@ -823,6 +825,9 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_mixed(DevFieldsMixed f, DevSca
         // every repetition of the region is evaluated: the inputs pass through
         // an empty asm statement at the top of each one, carried in the same
         // registers (see k_q_eos_dt)
+        // loads at wave priority 1, the repetitions at 0, the stores and the
+        // reduction at 1 (as in k_q_eos_dt)
+        __builtin_amdgcn_s_setprio(0);
         double a0 = vnewc, a1 = e_old;
         float a2 = delvc, a3 = p_old, a4 = q_old, a5 = qquad, a6 = qlin;
         for (int r = 0; r < rep; ++r) {
@@ -831,6 +836,7 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_mixed(DevFieldsMixed f, DevSca
             o = eos_element_mixed(a0, a1, a2, a3, a4, a5, a6, ec);
             asm volatile("" ::"v"(o.p), "v"(o.e), "v"(o.q), "v"(o.ss));
         }
+        __builtin_amdgcn_s_setprio(1);
         f.p[el] = o.p;
         f.e[el] = o.e;
         f.q[el] = o.q;
