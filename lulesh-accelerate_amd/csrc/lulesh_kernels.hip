@@ -142,7 +142,11 @@ __device__ __forceinline__ void time_increment(DevScalars *s, int ic_mode)
 __global__ void k_time_increment(DevScalars *s, int ic_mode)
 {
     if (threadIdx.x == 0) {
-        time_increment(s, ic_mode);
+        // on a copy in registers: one batch of loads and one of stores, not a
+        // wait for memory at every field the pointer form might alias
+        DevScalars t = *s;
+        time_increment(&t, ic_mode);
+        *s = t;
     }
 }
 
