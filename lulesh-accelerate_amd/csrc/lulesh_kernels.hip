@@ -156,7 +156,10 @@ __global__ void k_time_increment_batch(const BatchMember *__restrict__ members,
 {
     const int n = work->num_members;
     for (int m = int(threadIdx.x); m < n; m += int(blockDim.x)) {
-        time_increment(members[m].s, ic_mode);
+        DevScalars *const s = members[m].s;
+        DevScalars t = *s;  // on a register copy, as in k_time_increment
+        time_increment(&t, ic_mode);
+        *s = t;
     }
 }
 
