@@ -10,7 +10,8 @@ GPU; the timed region is K consecutive cycles after W warm-up cycles.
 Prints one JSON line: zones*cycles/s of the timed cycles (queued back to
 back, no events between kernels), the roofline of the dominant kernel
 (algorithmic bytes per launch over its mean duration, from HIP events
-around every launch on the domain's stream during 50 further cycles) and
+around every launch on the domain's stream during the first 50 cycles,
+which run before the warm-up and the timed cycles) and
 the reference LULESH 2.0.3 (oracle/_ref/lulesh2, built from the reference
 sources) timed on this host's cores on a 20-cycle sample of the same
 workload.
@@ -63,11 +64,30 @@ KERNEL_BYTES = {
 
 
 def host_cores() -> int:
+    """Threads for the CPU baseline: OMP_NUM_THREADS where the environment
+    sets it (the host's CPU share for this GPU), else the cores this
+    process may run on."""
     try:
-        n = len(os.sched_getaffinity(0))
-    except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        n = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        n = 0
+    if n < 1:
+        try:
+            n = len(os.sched_getaffinity(0))
+        except AttributeError:
+            n = os.cpu_count() or 1
+    return max(1, n)
+
+
+def host_cpu_model() -> str:
+    """The host CPU's model name (the lscpu "Model name" line)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(nx: int, cycles: int) -> dict | None:
@@ -86,7 +106,7 @@ def cpu_baseline(nx: int, cycles: int) -> dict | None:
         if out.returncode == 0 and m and c:
             grind_us = float(m.group(1))
             return {"value": 1.0e6 / grind_us, "unit": "zones·cycles/s",
-                    "cores": cores, "kind": "reference",
+                    "cores": cores, "cpu": host_cpu_model(), "kind": "reference",
                     "sample": (f"lulesh2 (reference/C/lulesh-2.0.3, g++ -O3 "
                                f"-fopenmp) -s {nx} -i {c.group(1)}, "
                                f"OMP_NUM_THREADS={cores}; reference's own "
@@ -99,7 +119,7 @@ def cpu_baseline(nx: int, cycles: int) -> dict | None:
     o.run(cycles)
     dt = time.perf_counter() - t0
     return {"value": nx ** 3 * o.cycle / dt, "unit": "zones·cycles/s",
-            "cores": cores, "kind": "port",
+            "cores": cores, "cpu": host_cpu_model(), "kind": "port",
             "sample": f"oracle restatement, {nx}^3, {o.cycle} cycles, "
                       f"{cores} OpenMP threads"}
 
@@ -358,9 +378,23 @@ def main() -> int:
     import lulesh_amd
 
     nx, K, W = args.nx, args.steps, args.warmup
-    KT = min(K, 50)  # cycles of the per-kernel breakdown, after the timed ones
+    KT = 50  # cycles of the per-kernel breakdown, before the warm-up
     d = lulesh_amd.Domain(nx, ic_mode=2, num_reg=11, balance=1, cost=1,
-                          max_cycles=W + K + KT)
+                          max_cycles=KT + W + K)
+    # Per-kernel durations first: HIP events around every launch on the
+    # domain's stream over KT cycles.  They also take the GPU from idle to
+    # its sustained clock: from idle, the first 40 cycles of this domain run
+    # up to 10% slower (the sweep 346 us in cycles 1-10, 291 us from cycle
+    # 40 on), and a GPU kept busy beforehand runs cycles 6-25 at the
+    # steady-state rate.
+    d.set_kernel_timing(True)
+    rc, done = d.step(KT)
+    if rc != 0 or done != KT:
+        print(f"bench: breakdown step returned rc={rc}, {done} of {KT} cycles",
+              file=sys.stderr)
+        return 1
+    times = d.kernel_times()
+    d.set_kernel_timing(False)
     if W:
         d.step(W)
     # the timed region: K cycles queued back to back, no events between the
@@ -374,15 +408,6 @@ def main() -> int:
               file=sys.stderr)
         return 1
     st = d.stats()
-    # per-kernel durations: HIP events around every launch on the domain's
-    # stream, over KT further cycles
-    d.set_kernel_timing(True)
-    rc, done = d.step(KT)
-    if rc != 0 or done != KT:
-        print(f"bench: breakdown step returned rc={rc}, {done} of {KT} cycles",
-              file=sys.stderr)
-        return 1
-    times = d.kernel_times()
     d.close()
 
     ne, nn = nx ** 3, (nx + 1) ** 3
@@ -412,7 +437,7 @@ def main() -> int:
         "config": {"workload": f"sedov_{nx}^3_lulesh2.0_regions11_balance1_cost1",
                    "nx": nx, "zones": ne, "nodes": nn, "ic": "LULESH 2.0.3",
                    "num_reg": 11, "balance": 1, "cost": 1,
-                   "cycles_timed": K, "first_timed_cycle": W + 1,
+                   "cycles_timed": K, "first_timed_cycle": KT + W + 1,
                    "final_cycle": st.cycle},
         "roofline": {"bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -425,7 +450,7 @@ def main() -> int:
                            "frac": value / roof_value},
         "kernels_mean_ms": {k: round(v, 6) for k, v in per_kernel.items()},
         "gpu_event_ms_per_step": sum(ms for ms, _ in times.values()) / KT,
-        "breakdown_cycles": KT,
+        "breakdown_cycles": [1, KT],
     }
     forms = [] if args.no_forms else [f for f in args.forms.split(",") if f]
     unknown = set(forms) - {"batched", "mixed", "learned_q", "unstructured"}

@@ -161,7 +161,10 @@ hipEvent_t next_event(lulesh_domain *d)
     return d->event_pool[d->events_used++];
 }
 
-// launches one kernel, bracketed by events when kernel timing is on
+// launches one kernel, bracketed by events when kernel timing is on.  A body
+// that returns a hipError_t (a launcher that checks its own launch, which
+// also clears the runtime's last error) has that value checked; any other
+// body is checked through hipGetLastError.
 template <typename F>
 int launch(lulesh_domain *d, int kid, F &&body)
 {
@@ -174,8 +177,19 @@ int launch(lulesh_domain *d, int kid, F &&body)
         }
         HIP_TRY(d, hipEventRecord(a, d->stream));
     }
-    body();
-    HIP_TRY(d, hipGetLastError());
+    hipError_t e = hipSuccess;
+    if constexpr (std::is_same_v<decltype(body()), hipError_t>) {
+        e = body();
+    } else {
+        body();
+    }
+    if (e == hipSuccess) {
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) {
+        return fail(d, LULESH_HIP_ERROR,
+                    std::string(kKernelNames[kid]) + " launch: " + hipGetErrorString(e));
+    }
     if (d->timing) {
         HIP_TRY(d, hipEventRecord(b, d->stream));
         d->pending.push_back({kid, a, b});
@@ -233,12 +247,9 @@ int queue_cycle(lulesh_domain *d)
              hipLaunchKernelGGL(k_time_increment, dim3(1), dim3(64), 0, st, s, p.ic_mode);
          })))
         return rc;
-    if ((rc = launch(d, KID_FORCE_SWEEP, [&] {
-             if (mixed) {
-                 (void)launch_force_sweep_mixed(d->sweep, fm, s, p, st);
-             } else {
-                 (void)launch_force_sweep(d->sweep, f, s, p, st);
-             }
+    if ((rc = launch(d, KID_FORCE_SWEEP, [&]() -> hipError_t {
+             return mixed ? launch_force_sweep_mixed(d->sweep, fm, s, p, st)
+                          : launch_force_sweep(d->sweep, f, s, p, st);
          })))
         return rc;
     if ((rc = launch(d, KID_KINEMATICS_Q, [&] {

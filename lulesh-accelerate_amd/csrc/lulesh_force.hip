@@ -345,7 +345,10 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
             coords(x, y, z);
             const double sig = -field(0) - field(1);
             const double det = jacobian_volume(x, y, z);
-            if (det <= 0.0) {
+            // VolumeError on the Jacobian volume, then on v, which the
+            // reference tests for every element before it looks at hgcoef
+            // (:1579-1584, :1540-1543)
+            if (det <= 0.0 || field(2) <= 0.0) {
                 atomicCAS(&s->error, 0, -1);
             }
             node_normals(pf[0], pf[1], pf[2], x, y, z);
@@ -363,9 +366,6 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
         if (hourglass) {
             if (compute) {
                 const double vv = field(2);
-                if (vv <= 0.0) {
-                    atomicCAS(&s->error, 0, -1);
-                }
                 const double determ = field(3) * vv;
                 const double volinv = 1.0 / determ;
                 // the coefficient before hourgam, which holds 32 values live
@@ -569,7 +569,10 @@ __global__ __launch_bounds__(256) void k_corner_forces_u(DevFields f, DevMeshU m
     {
         const double sig = -p_el - q_el;
         const double det = jacobian_volume(x, y, z);
-        if (det <= 0.0) {
+        // VolumeError on the Jacobian volume, then on v, which the reference
+        // tests for every element before it looks at hgcoef (:1579-1584,
+        // :1540-1543)
+        if (det <= 0.0 || vv <= 0.0) {
             atomicCAS(&s->error, 0, -1);
         }
         double pf[3][8];
@@ -584,9 +587,6 @@ __global__ __launch_bounds__(256) void k_corner_forces_u(DevFields f, DevMeshU m
     }
     if (!(prm.hgcoef > 0.0)) {
         return;
-    }
-    if (vv <= 0.0) {
-        atomicCAS(&s->error, 0, -1);
     }
     const double determ = volo * vv;
     const double volinv = 1.0 / determ;

@@ -320,10 +320,20 @@ MeshLists read_lmesh(const char *path)
         l.y[size_t(i)] = next_double();
         l.z[size_t(i)] = next_double();
     }
+    // indices are range-checked as the long strtol returns, before they are
+    // narrowed to int32 (a value such as 2^32 + 5 would otherwise wrap to a
+    // valid index)
+    auto next_index = [&](const char *what, long limit) -> int32_t {
+        const long v = next_int(what);
+        if (v < 0 || v >= limit) {
+            throw std::invalid_argument(std::string("lmesh: ") + what + " out of range");
+        }
+        return int32_t(v);
+    };
     l.nodelist.resize(size_t(ne) * 8);
-    for (auto &v : l.nodelist) v = int32_t(next_int("a node index"));
+    for (auto &v : l.nodelist) v = next_index("a node index", nn);
     l.neighbors.resize(size_t(ne) * 6);
-    for (auto &v : l.neighbors) v = int32_t(next_int("a neighbour index"));
+    for (auto &v : l.neighbors) v = next_index("a neighbour index", ne);
     std::vector<int32_t> *sets[4] = {&l.symm_x, &l.symm_y, &l.symm_z, &l.free_surf};
     for (auto *set : sets) {
         const long n = next_int("a node set size");
@@ -331,7 +341,7 @@ MeshLists read_lmesh(const char *path)
             throw std::invalid_argument("lmesh: node set size out of range");
         }
         set->resize(size_t(n));
-        for (auto &v : *set) v = int32_t(next_int("a node set entry"));
+        for (auto &v : *set) v = next_index("a node set entry", nn);
     }
     return l;
 }

@@ -277,7 +277,10 @@ __device__ __forceinline__ void sweep_tile_mixed(const DevFieldsMixed &f, DevSca
                 corner8(0, X);
                 corner8(1, Y);
                 corner8(2, Z);
-                if (jacobian_volume(X, Y, Z) <= 0.0) {
+                // VolumeError on the Jacobian volume, then on v, which the
+                // reference tests for every element before it looks at
+                // hgcoef (reference/C/LULESH-OMP.cc:1579-1584, :1540-1543)
+                if (jacobian_volume(X, Y, Z) <= 0.0 || ev <= 0.0) {
                     raise_error(s, -1);
                 }
                 local8(X, x);
@@ -301,9 +304,6 @@ __device__ __forceinline__ void sweep_tile_mixed(const DevFieldsMixed &f, DevSca
             __builtin_amdgcn_s_setprio(0);
             if (compute) {
                 const double vv = ev;
-                if (vv <= 0.0) {
-                    raise_error(s, -1);
-                }
                 const float determ = float(evolo * vv);
                 const float volinv = frcp(determ);
                 float hourgam[8][4];

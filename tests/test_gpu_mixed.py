@@ -12,6 +12,47 @@ pytestmark = pytest.mark.gpu
 ORIGIN_E_TOL = 1e-5
 
 
+def mixed_errors(ref, mix):
+    """The mixed run's errors against the float64 run of the same problem:
+    origin energy and total energy (sum e*m + 1/2 sum m_node |xd|^2),
+    relative; shock radius (centroid radius of the element with the largest
+    pressure, over the mesh and along the x axis), absolute; cycles."""
+    sr, sm = ref.stats(), mix.stats()
+    rr, rm = ref.shock_radius(), mix.shock_radius()
+    et_r, et_m = ref.total_energy(), mix.total_energy()
+    return {
+        "origin_energy_rel": abs(sm.e_origin - sr.e_origin) / abs(sr.e_origin),
+        "total_energy_rel": abs(et_m - et_r) / abs(et_r),
+        "shock_radius_abs": abs(rm["radius"] - rr["radius"]),
+        "shock_radius_x_axis_abs": abs(rm["radius_x_axis"] - rr["radius_x_axis"]),
+        "cycles": (sm.cycle, sr.cycle),
+    }
+
+
+def test_mixed_120_cubed_against_fp64(lulesh_gpu):
+    """The mixed-precision workload at its stated size: Sedov 120^3 (LULESH
+    2.0 ICs, 11 regions) to t = 1e-2 beside the float64 step.  Bounds: the
+    origin energy within 1e-5 relative (the form's promise); total energy,
+    shock radius and cycle count at about ten times what the form measures
+    (origin energy 2.5e-6, total energy 5.8e-8, shock radius 6.2e-9 and
+    5.8e-8 along x, same cycle count 4264), so that a regression fails."""
+    nx = 120
+    ref = lulesh_gpu.Domain(nx)
+    mix = lulesh_gpu.Domain(nx, mixed=True)
+    assert ref.run()[0] == 0
+    assert mix.run()[0] == 0
+    assert mix.stats().done == 1 and mix.stats().time >= 1.0e-2 * (1 - 1e-12)
+    err = mixed_errors(ref, mix)
+    print("mixed 120^3 errors against float64:", err)
+    assert err["origin_energy_rel"] <= ORIGIN_E_TOL, err
+    assert err["total_energy_rel"] <= 6e-7, err
+    assert err["shock_radius_abs"] <= 6e-8, err
+    assert err["shock_radius_x_axis_abs"] <= 6e-7, err
+    assert abs(err["cycles"][0] - err["cycles"][1]) <= 2, err
+    ref.close()
+    mix.close()
+
+
 @pytest.mark.parametrize("nx", [30, 45])
 def test_mixed_against_fp64_to_stoptime(lulesh_gpu, nx):
     ref = lulesh_gpu.Domain(nx)
@@ -19,6 +60,7 @@ def test_mixed_against_fp64_to_stoptime(lulesh_gpu, nx):
     mix = lulesh_gpu.Domain(nx, mixed=True)
     assert mix.mixed and not ref.mixed
     mix.run()
+    print(f"mixed {nx}^3 errors against float64:", mixed_errors(ref, mix))
     sr, sm = ref.stats(), mix.stats()
     assert sm.done == 1 and sm.time >= 1.0e-2 * (1 - 1e-12)
     assert abs(sm.cycle - sr.cycle) <= max(2, sr.cycle // 200), (sm.cycle, sr.cycle)

@@ -209,11 +209,23 @@ def test_30_cubed_against_oracle(lulesh_gpu, oracle, mode):
     d.close()
 
 
-def test_150_cubed_first_cycles_bitwise(lulesh_gpu, oracle):
-    """The benchmark configuration (150^3, 2.0 regions) for a few cycles."""
-    d, o = make_pair(lulesh_gpu, oracle, 2, 150, max_cycles=3)
-    assert d.run() == (0, 3)
-    assert o.run(3) == 0
+def test_150_cubed_benchmark_cycles_bitwise(lulesh_gpu, oracle):
+    """The benchmark configuration (150^3, 2.0 regions, 11 regions, balance
+    1, cost 1) bit for bit over every cycle the benchmark runs by default
+    with --steps 20 --warmup 5: 50 cycles of per-kernel timing, 5 warm-up
+    and 20 timed cycles, checked after cycle 3, 25 and 75.  max_cycles is the
+    reference's -i (the `cycle < its` loop test of
+    reference/C/lulesh-2.0.3/lulesh.cc:2771)."""
+    d, o = make_pair(lulesh_gpu, oracle, 2, 150, max_cycles=75)
     np.testing.assert_array_equal(d.regions(), o.get_int("regNumList"))
-    assert_bitwise(d, o, NODE + ELEM)
+    done = 0
+    for upto in (3, 25, 75):
+        assert d.step(upto - done) == (0, upto - done)
+        assert o.run(upto) == 0
+        assert d.stats().cycle == o.cycle == upto
+        assert_bitwise(d, o, NODE + ELEM)
+        for name in ("time", "deltatime", "dtcourant", "dthydro"):
+            assert bits_equal(getattr(d.stats(), name), o.scalar(name)), (upto, name)
+        done = upto
+    assert d.stats().done == 1
     d.close()

@@ -106,6 +106,33 @@ def test_tangled_mesh_volume_error(lulesh_gpu):
     d.close()
 
 
+@pytest.mark.parametrize("form", ["fp64", "mixed", "lists"])
+def test_nonpositive_volume_without_hourglass(lulesh_gpu, oracle, form):
+    """v <= 0 raises VolumeError also when hgcoef = 0: the reference tests v
+    of every element before it looks at hgcoef
+    (reference/C/LULESH-OMP.cc:1540-1548).  The EOS clamps v to eosvmin
+    before its own test, so without this check the run would go on."""
+    nx = 6
+    if form == "lists":
+        d = lulesh_gpu.Domain.from_mesh(oracle.cube_mesh(nx), max_cycles=10)
+        o = oracle.Oracle.from_mesh(oracle.cube_mesh(nx), cbrt_mode=1)
+    else:
+        d = lulesh_gpu.Domain(nx, ic_mode=2, mixed=(form == "mixed"), max_cycles=10)
+        o = oracle.Oracle(nx, mode=2, cbrt_mode=1)
+    for dom in (d, o):
+        if dom is d:
+            dom.set_parameter("hgcoef", 0.0)
+        else:
+            dom.set_scalar("hgcoef", 0.0)
+        v = dom.get("v")
+        v[40] = -0.25
+        dom.set("v", v)
+    rc, _ = d.step(1, raise_on_error=False)
+    assert o.run(1) == lulesh_gpu.VOLUME_ERROR
+    assert rc == lulesh_gpu.VOLUME_ERROR
+    d.close()
+
+
 def test_launch_schedule_does_not_change_results(lulesh_gpu):
     """The sweep's launch shape and the Q/EOS band interleave only move work
     between blocks: every choice gives the same bits."""

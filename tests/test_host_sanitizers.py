@@ -38,6 +38,19 @@ def test_host_code_under_sanitizers(tmp_path):
               "empty": "",
               "garbage": "this is not a mesh\n",
               "negative": "-5 8\n" + good.split("\n", 1)[1]}
+    # indices beyond int32 that wrap to a valid index when narrowed
+    # (2^32 + 5 -> 5): the first node index, the first neighbour index and
+    # the first entry of the x symmetry set
+    tok = good.split()
+    ne, nn = int(tok[0]), int(tok[1])
+    first_node = 2 + 3 * nn
+    first_nbr = first_node + 8 * ne
+    first_set = first_nbr + 6 * ne + 1
+    for name, pos in (("wrapped_node", first_node), ("wrapped_neighbour", first_nbr),
+                      ("wrapped_set", first_set)):
+        t = list(tok)
+        t[pos] = str((1 << 32) + int(t[pos]))
+        broken[name] = " ".join(t) + "\n"
     paths = [os.path.join(GOLDEN, "sedov15oct.lmesh"), str(big)]
     for name, text in broken.items():
         p = tmp_path / f"{name}.lmesh"
