@@ -327,6 +327,24 @@ int lulesh_mlp_eval(lulesh_mlp *n, const float *x, int count, float *y);
  * respect to the parameters (w = out_weight[2], NULL: 1), and the loss. */
 int lulesh_mlp_gradient(lulesh_mlp *n, const float *x, const float *t, int count,
                         const float *out_weight, float *grad, double *loss);
+/* lulesh_mlp_gradient without any allocation.  It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, training loops that keep their samples on the device,
+ * harnesses that forbid hipMalloc).  x[count][12], t[count][2], grad[params]
+ * and loss[1] (fp32) are DEVICE pointers; out_weight is a host array of 2
+ * (NULL: 1, 1).  The kernels are queued on `stream` (a hipStream_t; NULL: the
+ * null stream) and not waited for, and the call makes no allocation: the
+ * per-wave partial gradients go to a fixed buffer of the library, 5.0 MiB
+ * held on each GPU until the process ends, which bounds count to
+ * LULESH_MLP_NOALLOC_MAX (a larger count returns LULESH_BAD_ARG).  Every
+ * call on one GPU shares that buffer: calls that overlap on one GPU (other
+ * host threads, or other streams before this call's kernels finished) race.
+ * The current device must be the one the network was created on.  grad and
+ * loss are bit-identical to lulesh_mlp_gradient's for the same samples. */
+#define LULESH_MLP_NOALLOC_MAX 16384
+int lulesh_mlp_gradient_noalloc(lulesh_mlp *n, const float *x, const float *t, int count,
+                                const float *out_weight, float *grad, float *loss,
+                                void *stream);
 /* Adam (beta 0.9 / 0.999, eps 1e-8) on minibatches of `batch` samples drawn
  * without replacement (splitmix64 shuffle from `seed`), `epochs` passes;
  * *final_loss receives the mean minibatch loss of the last pass. */

@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -32,6 +33,17 @@ namespace {
 
 constexpr int kWaves = 4;  // waves per block
 constexpr int kBlockThreads = 64 * kWaves;
+
+// The partial gradients of lulesh_mlp_gradient_noalloc: one row of
+// kMlpParams + 1 floats per wave of 64 samples, for up to
+// LULESH_MLP_NOALLOC_MAX samples (a whole number of blocks); 5.0 MiB.  Each
+// GPU the library runs on holds its own copy until the process ends.  In
+// host code the name stands for no device address: host code uses the
+// per-device pointer that noalloc_scratch() resolves.
+constexpr int kNoallocWaves = LULESH_MLP_NOALLOC_MAX / 64;
+static_assert(LULESH_MLP_NOALLOC_MAX % kBlockThreads == 0,
+              "the no-allocation buffer holds whole blocks");
+static __device__ __align__(256) float scratch[size_t(kNoallocWaves) * (kMlpParams + 1)];
 
 __global__ void k_mlp_pack(const float *__restrict__ p, const float *__restrict__ norm,
                            MlpFrags *F)
@@ -355,6 +367,7 @@ struct lulesh_mlp {
     float *d_part = nullptr;
     size_t part_waves = 0;
     int step = 0;
+    int device = -1;  // the device current at lulesh_mlp_create
     hipStream_t stream = nullptr;
     std::string err;
 };
@@ -433,6 +446,32 @@ int run_batch(lulesh_mlp *n, const float *dx, const float *dt, const int *didx, 
     return LULESH_OK;
 }
 
+// The device address of `scratch` on the current device, resolved once per
+// device (hipGetSymbolAddress with that device current) and kept; nullptr
+// and the HIP error when it cannot be resolved.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_ptr[kMaxDevices] = {};
+
+float *noalloc_scratch(int device, hipError_t *err)
+{
+    *err = hipSuccess;
+    if (device < 0 || device >= kMaxDevices) {
+        *err = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        *err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (*err != hipSuccess) {
+            return nullptr;
+        }
+        g_scratch_ptr[device] = static_cast<float *>(ptr);
+    }
+    return g_scratch_ptr[device];
+}
+
 }  // namespace
 
 const MlpFrags *lulesh_mlp_device_frags(const lulesh_mlp *n) { return n ? n->d_frags : nullptr; }
@@ -468,7 +507,8 @@ int lulesh_mlp_create(const float *params, const float *in_mean, const float *in
         norm[kMlpIn + i] = 1.0f / in_std[i];
     }
     const size_t pb = kMlpParams * sizeof(float);
-    bool ok = hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) == hipSuccess &&
+    bool ok = hipGetDevice(&n->device) == hipSuccess &&
+              hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) == hipSuccess &&
               hipMalloc(&n->d_params, pb) == hipSuccess &&
               hipMalloc(&n->d_m, pb) == hipSuccess && hipMalloc(&n->d_v, pb) == hipSuccess &&
               hipMalloc(&n->d_grad, pb) == hipSuccess &&
@@ -565,6 +605,50 @@ int lulesh_mlp_gradient(lulesh_mlp *n, const float *x, const float *t, int count
     if (dx) (void)hipFree(dx);
     if (dt) (void)hipFree(dt);
     return rc;
+}
+
+int lulesh_mlp_gradient_noalloc(lulesh_mlp *n, const float *x, const float *t, int count,
+                                const float *out_weight, float *grad, float *loss,
+                                void *stream)
+{
+    if (!n || !x || !t || !grad || !loss || count < 1) {
+        return LULESH_BAD_ARG;
+    }
+    if (count > LULESH_MLP_NOALLOC_MAX) {
+        return mfail(n, LULESH_BAD_ARG,
+                     "lulesh_mlp_gradient_noalloc: " + std::to_string(count) +
+                         " samples exceed the maximum of " +
+                         std::to_string(LULESH_MLP_NOALLOC_MAX) + " per call");
+    }
+    int device = -1;
+    MHIP_TRY(n, hipGetDevice(&device));
+    if (device != n->device) {
+        return mfail(n, LULESH_BAD_ARG,
+                     "lulesh_mlp_gradient_noalloc: the current device is not the "
+                     "network's");
+    }
+    hipError_t e = hipSuccess;
+    float *part = noalloc_scratch(device, &e);
+    if (!part) {
+        return mfail(n, LULESH_HIP_ERROR,
+                     std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
+    const float w0 = out_weight ? out_weight[0] : 1.f;
+    const float w1 = out_weight ? out_weight[1] : 1.f;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    // the kernels of run_batch without an Adam step: every wave of the grid
+    // writes its whole row of partials (masked samples add zeros), and the
+    // sum reads exactly those rows, so no row has to be cleared first
+    const int waves = (count + 63) / 64;
+    const int blocks = (waves + kWaves - 1) / kWaves;
+    hipLaunchKernelGGL(k_mlp_grad, dim3(blocks), dim3(kBlockThreads), 0, st, n->d_frags, x,
+                       t, nullptr, count, 0, count, w0, w1, part);
+    MHIP_TRY(n, hipGetLastError());
+    hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpParams + 1 + 255) / 256), dim3(256), 0, st,
+                       part, blocks * kWaves, grad, n->d_params, n->d_m, n->d_v, n->step,
+                       0.f, 0.9f, 0.999f, 1e-8f, count, loss);
+    MHIP_TRY(n, hipGetLastError());
+    return LULESH_OK;
 }
 
 int lulesh_mlp_train(lulesh_mlp *n, const float *x, const float *t, int count,

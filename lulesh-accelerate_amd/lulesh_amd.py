@@ -77,6 +77,7 @@ EXPORTS = (
     "lulesh_batch_last_error",
     "lulesh_mlp_param_count", "lulesh_mlp_create", "lulesh_mlp_destroy",
     "lulesh_mlp_get_params", "lulesh_mlp_eval", "lulesh_mlp_gradient",
+    "lulesh_mlp_gradient_noalloc",
     "lulesh_mlp_train", "lulesh_mlp_last_error", "lulesh_set_learned_q",
     "lulesh_q_features", "lulesh_write_domain_arrays", "lulesh_write_domain",
     "lulesh_create_lmesh", "lulesh_create_mesh", "lulesh_origin_element",
@@ -201,6 +202,8 @@ def lib() -> ctypes.CDLL:
     L.lulesh_mlp_eval.restype = I
     L.lulesh_mlp_gradient.argtypes = [P, FP, FP, I, FP, FP, DP]
     L.lulesh_mlp_gradient.restype = I
+    L.lulesh_mlp_gradient_noalloc.argtypes = [P, P, P, I, FP, P, P, P]
+    L.lulesh_mlp_gradient_noalloc.restype = I
     L.lulesh_mlp_train.argtypes = [P, FP, FP, I, FP, I, I, ctypes.c_float,
                                    ctypes.c_ulonglong, DP]
     L.lulesh_mlp_train.restype = I
@@ -707,6 +710,17 @@ class Mlp:
         self._check(lib().lulesh_mlp_gradient(self.h, _fp(x), _fp(t), len(x), _fp(w),
                                               _fp(g), ctypes.byref(loss)))
         return g, loss.value
+
+    def gradient_noalloc(self, x_dev: int, t_dev: int, count: int, grad_dev: int,
+                         loss_dev: int, stream: int = 0,
+                         out_weight=(1.0, 1.0)) -> None:
+        """lulesh_mlp_gradient_noalloc on device buffers given by address
+        (x[count][12], t[count][2], grad[params], loss[1], float32), queued
+        on `stream` (a hipStream_t handle, 0 = the null stream)."""
+        w = np.ascontiguousarray(out_weight, np.float32)
+        self._check(lib().lulesh_mlp_gradient_noalloc(
+            self.h, x_dev, t_dev, int(count), _fp(w), grad_dev, loss_dev,
+            stream or None))
 
     def train(self, x, t, epochs: int, batch: int = 1024, lr: float = 1e-3,
               seed: int = 0, out_weight=(1.0, 1.0)) -> float:
