@@ -10,8 +10,8 @@ GPU; the timed region is K consecutive cycles after W warm-up cycles.
 Prints one JSON line: zones*cycles/s of the timed cycles (queued back to
 back, no events between kernels), the roofline of the dominant kernel
 (algorithmic bytes per launch over its mean duration, from HIP events
-around every launch on the domain's stream during the first 50 cycles,
-which run before the warm-up and the timed cycles) and
+around every launch on the domain's stream during cycles 26-50, which run
+before the warm-up and the timed cycles) and
 the reference LULESH 2.0.3 (oracle/_ref/lulesh2, built from the reference
 sources) timed on this host's cores on a 20-cycle sample of the same
 workload.
@@ -378,15 +378,20 @@ def main() -> int:
     import lulesh_amd
 
     nx, K, W = args.nx, args.steps, args.warmup
-    KT = 50  # cycles of the per-kernel breakdown, before the warm-up
+    # Before the warm-up: KR cycles that take the GPU from idle to its
+    # sustained clock, then the per-kernel breakdown (HIP events around
+    # every launch on the domain's stream) over KT cycles.  From idle, the
+    # first 40 cycles of this domain run up to 10% slower (the sweep 346 us
+    # in cycles 1-10, 291 us from cycle 40 on); a GPU kept busy beforehand
+    # runs cycles 6-25 at the steady-state rate.  The events cost about 1%
+    # of a cycle, so they stay out of the timed cycles.
+    KR, KT = 25, 25
     d = lulesh_amd.Domain(nx, ic_mode=2, num_reg=11, balance=1, cost=1,
-                          max_cycles=KT + W + K)
-    # Per-kernel durations first: HIP events around every launch on the
-    # domain's stream over KT cycles.  They also take the GPU from idle to
-    # its sustained clock: from idle, the first 40 cycles of this domain run
-    # up to 10% slower (the sweep 346 us in cycles 1-10, 291 us from cycle
-    # 40 on), and a GPU kept busy beforehand runs cycles 6-25 at the
-    # steady-state rate.
+                          max_cycles=KR + KT + W + K)
+    rc, done = d.step(KR)
+    if rc != 0 or done != KR:
+        print(f"bench: step returned rc={rc}, {done} of {KR} cycles", file=sys.stderr)
+        return 1
     d.set_kernel_timing(True)
     rc, done = d.step(KT)
     if rc != 0 or done != KT:
@@ -437,7 +442,7 @@ def main() -> int:
         "config": {"workload": f"sedov_{nx}^3_lulesh2.0_regions11_balance1_cost1",
                    "nx": nx, "zones": ne, "nodes": nn, "ic": "LULESH 2.0.3",
                    "num_reg": 11, "balance": 1, "cost": 1,
-                   "cycles_timed": K, "first_timed_cycle": KT + W + 1,
+                   "cycles_timed": K, "first_timed_cycle": KR + KT + W + 1,
                    "final_cycle": st.cycle},
         "roofline": {"bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -450,7 +455,7 @@ def main() -> int:
                            "frac": value / roof_value},
         "kernels_mean_ms": {k: round(v, 6) for k, v in per_kernel.items()},
         "gpu_event_ms_per_step": sum(ms for ms, _ in times.values()) / KT,
-        "breakdown_cycles": [1, KT],
+        "breakdown_cycles": [KR + 1, KR + KT],
     }
     forms = [] if args.no_forms else [f for f in args.forms.split(",") if f]
     unknown = set(forms) - {"batched", "mixed", "learned_q", "unstructured"}

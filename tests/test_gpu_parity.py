@@ -212,8 +212,9 @@ def test_30_cubed_against_oracle(lulesh_gpu, oracle, mode):
 def test_150_cubed_benchmark_cycles_bitwise(lulesh_gpu, oracle):
     """The benchmark configuration (150^3, 2.0 regions, 11 regions, balance
     1, cost 1) bit for bit over every cycle the benchmark runs by default
-    with --steps 20 --warmup 5: 50 cycles of per-kernel timing, 5 warm-up
-    and 20 timed cycles, checked after cycle 3, 25 and 75.  max_cycles is the
+    with --steps 20 --warmup 5: 25 cycles to reach the GPU's sustained
+    clock, 25 of per-kernel timing, 5 warm-up and 20 timed cycles, checked
+    after cycle 3, 25 and 75.  max_cycles is the
     reference's -i (the `cycle < its` loop test of
     reference/C/lulesh-2.0.3/lulesh.cc:2771)."""
     d, o = make_pair(lulesh_gpu, oracle, 2, 150, max_cycles=75)
