@@ -268,6 +268,8 @@ int queue_cycle(lulesh_domain *d)
              } else if (d->learned) {
                  hipLaunchKernelGGL(k_q_eos_dt_learned, dim3(ge), dim3(kBlock), 0, st, f, s,
                                     p, d->learned, d->learned_z_max);
+             } else if (d->eos_bands == 0) {
+                 hipLaunchKernelGGL(k_q_eos_dt_nat, dim3(ge), dim3(kBlock), 0, st, f, s, p);
              } else {
                  hipLaunchKernelGGL(k_q_eos_dt, dim3(ge), dim3(kBlock), 0, st, f, s, p,
                                     d->eos_bands);
@@ -424,6 +426,7 @@ int create_domain(const lulesh_config *cfg, bool mixed, double efactor,
     d->cfg = *cfg;
     d->mixed = mixed;
     d->sweep = mixed ? kDefaultSweepMixed : kDefaultSweep;
+    d->eos_bands = mixed ? kDefaultEosBands : 0;
     if (!mixed && cfg->nx >= 1 && cfg->nx <= 600) {
         d->sweep.nz = default_sweep_planes(cfg->nx);
     }
@@ -546,6 +549,7 @@ int build_device_domain_u(lulesh_domain *d, const HostMeshU &m)
     if ((rc = upload_vec(d, order, &ord)) || (rc = upload_vec(d, rep, &rp))) return rc;
     f.eos_order = ord;
     f.eos_rep = rp;
+    f.rep = rp;  // the identity order: the same array in mesh order
     DevMeshU &g = d->mesh;
     int32_t *a = nullptr;
     if ((rc = upload_vec(d, m.nodelist, &a))) return rc;
@@ -945,8 +949,11 @@ int lulesh_set_parameter(lulesh_domain *d, const char *name, double value)
 
 int lulesh_set_eos_bands(lulesh_domain *d, int bands)
 {
-    if (!d || bands < 1 || bands > 4096) {
+    if (!d || bands < 0 || bands > 4096) {
         return LULESH_BAD_ARG;
+    }
+    if (bands == 0 && d->mixed) {
+        return fail(d, LULESH_BAD_ARG, "the mixed-precision Q/EOS has no mesh-order schedule");
     }
     d->eos_bands = bands;
     return LULESH_OK;

@@ -71,6 +71,18 @@ __device__ __forceinline__ void apply_constraints(DevScalars *s, int ic_mode)
     if (h < kBig) s->dthydro = h;
 }
 
+// Block b of g in the mesh order the blocks of a natural-order kernel walk:
+// the hardware deals the blocks of a launch out to the 8 XCDs in turn
+// (block b to XCD b % 8), so XCD r takes the r-th of 8 contiguous ranges of
+// blocks, in order, and the planes one element layer shares with the next
+// stay in that XCD's L2.  A bijection of [0, g).
+__device__ __forceinline__ int xcd_contiguous_block(int b, int g)
+{
+    const int q = g >> 3, rem = g & 7;
+    const int r = b & 7, i = b >> 3;
+    return r * q + min(r, rem) + i;
+}
+
 // element (i, j, k) of a flat element index
 __device__ __forceinline__ void elem_ijk(const DevFields &f, int el, int &i, int &j,
                                          int &k)
@@ -515,6 +527,120 @@ __device__ __forceinline__ void monotonic_q(const PhysParams &prm, const QInputs
     }
 }
 
+// What one EOS evaluation of an element reads (the clamped new volume and
+// the values of the last cycle), formed once per element.
+struct EosIn {
+    double vnewc, e_old, delvc, p_old, q_old, qquad, qlin;
+};
+
+__device__ __forceinline__ EosConst eos_const(const PhysParams &prm)
+{
+    EosConst ec;
+    ec.e_cut = prm.e_cut;
+    ec.p_cut = prm.p_cut;
+    ec.q_cut = prm.q_cut;
+    ec.eosvmax = prm.eosvmax;
+    ec.eosvmin = prm.eosvmin;
+    ec.pmin = prm.pmin;
+    ec.emin = prm.emin;
+    ec.rho0 = prm.refdens;
+    ec.unit_rho0 = prm.refdens == 1.0;
+    ec.ss_tiny = prm.ss_tiny;
+    ec.ss_floor = prm.ss_floor;
+    return ec;
+}
+
+// The element's side of the EOS before its evaluations: ql / qq stored,
+// qstop tested on the q of the previous cycle (:2368-2381), the new and old
+// relative volumes clamped (:2704-2753; 2.0 clamps vnew in place) and the
+// clamped old volume tested.
+__device__ __forceinline__ EosIn eos_prepare(const DevFields &f, DevScalars *s,
+                                             const PhysParams &prm, int el,
+                                             const QInputs &in, double qlin, double qquad)
+{
+    f.ql[el] = qlin;
+    f.qq[el] = qquad;
+    const double q_old = in.q_old;
+    if (q_old > prm.qstop) {
+        raise_error(s, -2);
+    }
+    double vnewc = in.vnew;
+    if (prm.eosvmin != 0.) {
+        if (vnewc < prm.eosvmin) vnewc = prm.eosvmin;
+    }
+    if (prm.eosvmax != 0.) {
+        if (vnewc > prm.eosvmax) vnewc = prm.eosvmax;
+    }
+    double vc = in.v_old;
+    if (prm.eosvmin != 0.) {
+        if (vc < prm.eosvmin) vc = prm.eosvmin;
+    }
+    if (prm.eosvmax != 0.) {
+        if (vc > prm.eosvmax) vc = prm.eosvmax;
+    }
+    if (vc <= 0.) {
+        raise_error(s, -1);
+    }
+    return EosIn{vnewc, in.e_old, in.delv, in.p_old, q_old, qquad, qlin};
+}
+
+// `rep` evaluations of the EOS on the same inputs, as the reference repeats
+// them (lulesh-2.0.3/lulesh.cc:2267, :2413-2424).  The inputs pass through an
+// empty asm statement at the top of every repetition, so the compiler cannot
+// know they are unchanged and computes each repetition in full; they are
+// carried from one repetition to the next in the same registers (no copies),
+// and every repetition's result is marked as used.
+template <bool Unit>
+__device__ __forceinline__ EosOut eos_repeat(const EosIn &a, int rep, const EosConst &ec)
+{
+    double a0 = a.vnewc, a1 = a.e_old, a2 = a.delvc, a3 = a.p_old, a4 = a.q_old;
+    double a5 = a.qquad, a6 = a.qlin;
+    EosOut o;
+    o.p = o.e = o.q = o.ss = 0.0;
+    for (int r = 0; r < rep; ++r) {
+        asm volatile("" : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(a4), "+v"(a5),
+                     "+v"(a6));
+        o = eos_element<Unit>(a0, a1, a2, a3, a4, a5, a6, ec);
+        asm volatile("" ::"v"(o.p), "v"(o.e), "v"(o.q), "v"(o.ss));
+    }
+    return o;
+}
+
+// The element's side after the EOS: p, e, q, ss stored, the volume update
+// (UpdateVolumesForElems: 1.0 uses the unclamped vnew, 2.0 the clamped one,
+// lulesh-2.0.3/lulesh.cc:2439-2452) and the Courant and hydro candidates
+// (:2799-2933).
+__device__ __forceinline__ void eos_finish(const DevFields &f, const PhysParams &prm, int el,
+                                           const QInputs &in, double vnewc, const EosOut &o,
+                                           unsigned long long &cand_c,
+                                           unsigned long long &cand_h)
+{
+    const double vdov = in.vdov, vnew = in.vnew;
+    f.p[el] = o.p;
+    f.e[el] = o.e;
+    f.q[el] = o.q;
+    f.ss[el] = o.ss;
+    double tmpv = (prm.ic_mode == 1) ? vnew : vnewc;
+    if (fabs(tmpv - 1.0) < prm.v_cut) tmpv = 1.0;
+    f.v[el] = tmpv;
+    if (prm.ic_mode == 2) {
+        f.vnew[el] = vnewc;
+    }
+    if (vdov != 0.) {
+        const double arealg = in.arealg;
+        const double qqc2 = 64.0 * prm.qqc * prm.qqc;
+        double dtf = o.ss * o.ss;
+        if (vdov < 0.) {
+            dtf = dtf + qqc2 * arealg * arealg * vdov * vdov;
+        }
+        dtf = sqrt(dtf);
+        dtf = arealg / dtf;
+        cand_c = as_bits(dtf);
+        const double dtdvov = prm.dvovmax / (fabs(vdov) + 1.e-20);
+        cand_h = as_bits(dtdvov);
+    }
+}
+
 // Everything after ql / qq for one element: qstop, the volume clamps, the
 // EOS with the region repetitions, the volume update and the element's
 // Courant / hydro candidates.
@@ -524,105 +650,17 @@ __device__ __forceinline__ void eos_dt_elem(const DevFields &f, DevScalars *s,
                                             unsigned long long &cand_c,
                                             unsigned long long &cand_h)
 {
-    const double vdov = in.vdov, vnew = in.vnew;
-    {
-        f.ql[el] = qlin;
-        f.qq[el] = qquad;
-
-        // qstop is tested on the q of the previous cycle (:2368-2381)
-        const double q_old = in.q_old;
-        if (q_old > prm.qstop) {
-            raise_error(s, -2);
-        }
-
-        // clamp the new relative volume (:2704-2753; 2.0 clamps in place)
-        double vnewc = vnew;
-        if (prm.eosvmin != 0.) {
-            if (vnewc < prm.eosvmin) vnewc = prm.eosvmin;
-        }
-        if (prm.eosvmax != 0.) {
-            if (vnewc > prm.eosvmax) vnewc = prm.eosvmax;
-        }
-        double vc = in.v_old;
-        if (prm.eosvmin != 0.) {
-            if (vc < prm.eosvmin) vc = prm.eosvmin;
-        }
-        if (prm.eosvmax != 0.) {
-            if (vc > prm.eosvmax) vc = prm.eosvmax;
-        }
-        if (vc <= 0.) {
-            raise_error(s, -1);
-        }
-
-        EosConst ec;
-        ec.e_cut = prm.e_cut;
-        ec.p_cut = prm.p_cut;
-        ec.q_cut = prm.q_cut;
-        ec.eosvmax = prm.eosvmax;
-        ec.eosvmin = prm.eosvmin;
-        ec.pmin = prm.pmin;
-        ec.emin = prm.emin;
-        ec.rho0 = prm.refdens;
-        ec.unit_rho0 = prm.refdens == 1.0;
-        ec.ss_tiny = prm.ss_tiny;
-        ec.ss_floor = prm.ss_floor;
-        const double e_old = in.e_old, delvc = in.delv, p_old = in.p_old;
-        EosOut o;
-        o.p = o.e = o.q = o.ss = 0.0;
-        // the region's repetitions redo the same evaluation, as the
-        // reference does (lulesh-2.0.3/lulesh.cc:2267).  The inputs pass
-        // through an empty asm statement at the top of every repetition, so
-        // the compiler cannot know they are unchanged and computes each
-        // repetition in full; they are carried from one repetition to the
-        // next in the same registers (no copies), and every repetition's
-        // result is marked as used.
-        auto repeat = [&](auto unit) {
-            constexpr bool kUnit = decltype(unit)::value;
-            double a0 = vnewc, a1 = e_old, a2 = delvc, a3 = p_old, a4 = q_old;
-            double a5 = qquad, a6 = qlin;
-            for (int r = 0; r < rep; ++r) {
-                asm volatile("" : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(a4),
-                             "+v"(a5), "+v"(a6));
-                o = eos_element<kUnit>(a0, a1, a2, a3, a4, a5, a6, ec);
-                asm volatile("" ::"v"(o.p), "v"(o.e), "v"(o.q), "v"(o.ss));
-            }
-        };
-        __builtin_amdgcn_s_setprio(0);  // the repetitions yield to waves still loading
-        if (ec.unit_rho0) {
-            repeat(std::true_type{});
-        } else {
-            repeat(std::false_type{});
-        }
-        __builtin_amdgcn_s_setprio(1);  // stores and the block's reduction
-        f.p[el] = o.p;
-        f.e[el] = o.e;
-        f.q[el] = o.q;
-        f.ss[el] = o.ss;
-
-        // UpdateVolumesForElems: 1.0 uses the unclamped vnew, 2.0 the
-        // clamped one (lulesh-2.0.3/lulesh.cc:2439-2452)
-        double tmpv = (prm.ic_mode == 1) ? vnew : vnewc;
-        if (fabs(tmpv - 1.0) < prm.v_cut) tmpv = 1.0;
-        f.v[el] = tmpv;
-        if (prm.ic_mode == 2) {
-            f.vnew[el] = vnewc;
-        }
-
-        // Courant and hydro constraints (:2799-2933)
-        if (vdov != 0.) {
-            const double arealg = in.arealg;
-            const double qqc2 = 64.0 * prm.qqc * prm.qqc;
-            double dtf = o.ss * o.ss;
-            if (vdov < 0.) {
-                dtf = dtf + qqc2 * arealg * arealg * vdov * vdov;
-            }
-            dtf = sqrt(dtf);
-            dtf = arealg / dtf;
-            cand_c = as_bits(dtf);
-            const double dtdvov = prm.dvovmax / (fabs(vdov) + 1.e-20);
-            cand_h = as_bits(dtdvov);
-        }
+    const EosIn a = eos_prepare(f, s, prm, el, in, qlin, qquad);
+    const EosConst ec = eos_const(prm);
+    EosOut o;
+    __builtin_amdgcn_s_setprio(0);  // the repetitions yield to waves still loading
+    if (ec.unit_rho0) {
+        o = eos_repeat<true>(a, rep, ec);
+    } else {
+        o = eos_repeat<false>(a, rep, ec);
     }
+    __builtin_amdgcn_s_setprio(1);  // stores and the block's reduction
+    eos_finish(f, prm, el, in, a.vnewc, o, cand_c, cand_h);
 }
 
 // fold the block minima into the cycle's accumulators; they are applied to
@@ -683,6 +721,126 @@ __global__ __launch_bounds__(256) void k_q_eos_dt(DevFields f, DevScalars *s,
     const int chunk = eos_chunk(int(blockIdx.x), int(gridDim.x), bands);
     __builtin_amdgcn_s_setprio(1);  // the element's loads first
     q_eos_dt_block(f, s, prm, chunk * int(blockDim.x) + int(threadIdx.x), lds_c, lds_h);
+}
+
+namespace {
+
+// Per wave: the EOS inputs of each lane's element and the inclusive prefix of
+// the evaluations beyond each element's first.
+struct RepLds {
+    double in[7][64];
+    int end[64];
+};
+
+// The EOS evaluations of a wave's 64 elements beyond each one's first.  The
+// lane that owns an element evaluates it once (its result is the element's);
+// the other rep - 1 evaluations of every element are dealt out over all 64
+// lanes, so a wave of elements with 1, 2 and 20 repetitions takes
+// ceil(sum(rep - 1) / 64) extra passes instead of running every lane 20
+// times.  An element's evaluations all read the same inputs and give the
+// same results: each is computed in full (its inputs pass through an empty
+// asm statement) and its result marked as used, as the reference repeats the
+// evaluation (lulesh-2.0.3/lulesh.cc:2413-2424).  Every lane of the wave
+// calls it.
+template <bool Unit>
+__device__ __forceinline__ void eos_extra_evaluations(const EosIn &a, int extra,
+                                                      const EosConst &ec, RepLds &L)
+{
+    const int lane = int(threadIdx.x) & 63;
+    int incl = extra;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += o;
+    }
+    const int total = __shfl(incl, 63, 64);
+    if (total == 0) {
+        return;
+    }
+    L.end[lane] = incl;
+    L.in[0][lane] = a.vnewc;
+    L.in[1][lane] = a.e_old;
+    L.in[2][lane] = a.delvc;
+    L.in[3][lane] = a.p_old;
+    L.in[4][lane] = a.q_old;
+    L.in[5][lane] = a.qquad;
+    L.in[6][lane] = a.qlin;
+    // the wave reads what its lanes wrote: LDS operations of one wave are
+    // performed in order, and this keeps the compiler from moving them
+    __builtin_amdgcn_wave_barrier();
+    for (int w = lane; w < total; w += 64) {
+        // the lane whose evaluations [end - extra, end) hold number w: the
+        // count of lanes whose range ends at or before w (end is
+        // non-decreasing and end[63] = total > w, so src <= 63)
+        int src = 0;
+#pragma unroll
+        for (int step = 32; step >= 1; step >>= 1) {
+            if (L.end[src + step - 1] <= w) src += step;
+        }
+        double b0 = L.in[0][src], b1 = L.in[1][src], b2 = L.in[2][src], b3 = L.in[3][src];
+        double b4 = L.in[4][src], b5 = L.in[5][src], b6 = L.in[6][src];
+        asm volatile("" : "+v"(b0), "+v"(b1), "+v"(b2), "+v"(b3), "+v"(b4), "+v"(b5),
+                     "+v"(b6));
+        const EosOut o = eos_element<Unit>(b0, b1, b2, b3, b4, b5, b6, ec);
+        asm volatile("" ::"v"(o.p), "v"(o.e), "v"(o.q), "v"(o.ss));
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+}  // namespace
+
+// Monotonic Q, EOS, volume update and time constraints with the elements in
+// mesh order, one per thread: the limiter's neighbour values and the
+// element's own fields are read in the order they lie in memory, and the
+// blocks of each XCD walk one contiguous part of the mesh.  The regions'
+// EOS repetitions are balanced within each wave (eos_extra_evaluations).
+__global__ __launch_bounds__(256) void k_q_eos_dt_nat(DevFields f, DevScalars *s,
+                                                      PhysParams prm)
+{
+    __shared__ unsigned long long lds_c[4], lds_h[4];
+    __shared__ int skip;
+    __shared__ RepLds rep_lds[4];
+    if (threadIdx.x == 0) {
+        skip = (s->active == 0 || s->error != 0) ? 1 : 0;
+    }
+    __syncthreads();
+    if (skip) {
+        return;
+    }
+    __builtin_amdgcn_s_setprio(1);  // the element's loads first
+    const int b = xcd_contiguous_block(int(blockIdx.x), int(gridDim.x));
+    const int el = b * int(blockDim.x) + int(threadIdx.x);
+    const bool valid = el < f.num_elem;
+    unsigned long long cand_c = as_bits(kBig), cand_h = as_bits(kBig);
+    const EosConst ec = eos_const(prm);
+    QInputs in;
+    EosIn a{1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    EosOut o;
+    int extra = 0;
+    if (valid) {
+        const int rep = f.rep[el];
+        extra = rep - 1;
+        int i, j, k;
+        elem_ijk(f, el, i, j, k);
+        q_inputs(f, prm, el, i, j, k, in);
+        double qlin, qquad;
+        monotonic_q(prm, in, qlin, qquad);
+        a = eos_prepare(f, s, prm, el, in, qlin, qquad);
+    }
+    __builtin_amdgcn_s_setprio(0);  // the evaluations yield to waves still loading
+    RepLds &L = rep_lds[threadIdx.x >> 6];
+    if (ec.unit_rho0) {
+        o = eos_repeat<true>(a, 1, ec);
+        eos_extra_evaluations<true>(a, extra, ec, L);
+    } else {
+        o = eos_repeat<false>(a, 1, ec);
+        eos_extra_evaluations<false>(a, extra, ec, L);
+    }
+    __builtin_amdgcn_s_setprio(1);  // stores and the block's reduction
+    if (valid) {
+        eos_finish(f, prm, el, in, a.vnewc, o, cand_c, cand_h);
+    }
+    fold_minima(s, cand_c, cand_h, lds_c, lds_h);
 }
 
 // persistent blocks over the element chunks of all members; a chunk lies

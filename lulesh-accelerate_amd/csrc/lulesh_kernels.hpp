@@ -112,6 +112,8 @@ struct DevFields {
     // region-ordered EOS schedule
     const int32_t *eos_order;
     const uint16_t *eos_rep;
+    // EOS repetitions of each element, in mesh order
+    const uint16_t *rep;
 };
 
 // The mixed-precision domain: the node arrays, the energy, the volumes and
@@ -129,6 +131,7 @@ struct DevFieldsMixed {
     float *delv_xi, *delv_eta, *delv_zeta, *delx_xi, *delx_eta, *delx_zeta;
     const int32_t *eos_order;
     const uint16_t *eos_rep;
+    const uint16_t *rep;
 };
 
 // Launch shape of the force sweep: TX x TY element columns per block (one
@@ -182,6 +185,7 @@ struct BatchWork {
 __global__ void k_time_increment(DevScalars *s, int ic_mode);
 __global__ void k_kinematics_q(DevFields f, DevScalars *s, PhysParams prm);
 __global__ void k_q_eos_dt(DevFields f, DevScalars *s, PhysParams prm, int bands);
+__global__ void k_q_eos_dt_nat(DevFields f, DevScalars *s, PhysParams prm);
 __global__ void k_dt_finish(DevScalars *s, int ic_mode);
 __global__ void k_device_math(int op, int n, const double *a, const double *b,
                               double *out);

@@ -261,6 +261,7 @@ HostMesh build_mesh(const MeshConfig &cfg)
     }
     m.eos_order.resize(ne);
     m.eos_rep.resize(ne);
+    m.elem_rep.resize(ne);
     std::vector<int32_t> fill(m.region_offset.begin(), m.region_offset.end() - 1);
     for (size_t i = 0; i < ne; ++i) {
         const int r = m.reg_num_list[i] - 1;
@@ -268,6 +269,7 @@ HostMesh build_mesh(const MeshConfig &cfg)
         m.eos_order[static_cast<size_t>(slot)] = static_cast<int32_t>(i);
         m.eos_rep[static_cast<size_t>(slot)] = static_cast<uint16_t>(
             region_rep(cfg.ic_mode, m.num_reg, cfg.cost, r));
+        m.elem_rep[i] = m.eos_rep[static_cast<size_t>(slot)];
     }
     return m;
 }

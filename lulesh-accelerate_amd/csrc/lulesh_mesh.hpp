@@ -38,6 +38,7 @@ struct HostMesh {
     std::vector<int32_t> eos_order;     // elements sorted by region, ascending
     std::vector<uint16_t> eos_rep;      // EOS repetitions at each eos_order slot
     std::vector<int32_t> region_offset; // start of each region in eos_order
+    std::vector<uint16_t> elem_rep;     // EOS repetitions of each element
 };
 
 // Builds the mesh, masses, ICs and regions; throws std::invalid_argument.

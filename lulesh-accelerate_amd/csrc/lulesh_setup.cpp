@@ -52,7 +52,7 @@ size_t domain_arena_bytes(const HostMesh &m)
     const size_t pe = round_up(size_t(m.num_elem), 64);
     const size_t pn = round_up(size_t(m.num_node), 64);
     size_t bytes = (kNodeArrays * pn + kElemArrays * pe) * sizeof(double);
-    bytes += round_up(pe * sizeof(int32_t), 256) + round_up(pe * sizeof(uint16_t), 256);
+    bytes += round_up(pe * sizeof(int32_t), 256) + 2 * round_up(pe * sizeof(uint16_t), 256);
     bytes += round_up(sizeof(DevScalars), 256);
     return bytes;
 }
@@ -85,6 +85,8 @@ void carve_domain(char *base, const HostMesh &m, DevFields &f, DevScalars **s)
     sched += round_up(pe * sizeof(int32_t), 256);
     f.eos_rep = reinterpret_cast<uint16_t *>(sched);
     sched += round_up(pe * sizeof(uint16_t), 256);
+    f.rep = reinterpret_cast<uint16_t *>(sched);
+    sched += round_up(pe * sizeof(uint16_t), 256);
     *s = reinterpret_cast<DevScalars *>(sched);
 }
 
@@ -104,6 +106,7 @@ hipError_t upload_mesh(const HostMesh &m, const DevFields &f, hipStream_t stream
         {f.v, ones.data(), ne * sizeof(double)},
         {const_cast<int32_t *>(f.eos_order), m.eos_order.data(), ne * sizeof(int32_t)},
         {const_cast<uint16_t *>(f.eos_rep), m.eos_rep.data(), ne * sizeof(uint16_t)},
+        {const_cast<uint16_t *>(f.rep), m.elem_rep.data(), ne * sizeof(uint16_t)},
     };
     for (const Copy &c : copies) {
         const hipError_t e =
@@ -144,7 +147,7 @@ size_t domain_arena_bytes_mixed(const HostMesh &m)
     const size_t pn = round_up(size_t(m.num_node), 64);
     size_t bytes = (kNodeArrays * pn + kElemArrays64 * pe) * sizeof(double) +
                    kElemArrays32 * pe * sizeof(float);
-    bytes += round_up(pe * sizeof(int32_t), 256) + round_up(pe * sizeof(uint16_t), 256);
+    bytes += round_up(pe * sizeof(int32_t), 256) + 2 * round_up(pe * sizeof(uint16_t), 256);
     bytes += round_up(sizeof(DevScalars), 256);
     return bytes;
 }
@@ -178,6 +181,8 @@ void carve_domain_mixed(char *base, const HostMesh &m, DevFieldsMixed &f, DevSca
     sched += round_up(pe * sizeof(int32_t), 256);
     f.eos_rep = reinterpret_cast<uint16_t *>(sched);
     sched += round_up(pe * sizeof(uint16_t), 256);
+    f.rep = reinterpret_cast<uint16_t *>(sched);
+    sched += round_up(pe * sizeof(uint16_t), 256);
     *s = reinterpret_cast<DevScalars *>(sched);
 }
 
@@ -197,6 +202,7 @@ hipError_t upload_mesh_mixed(const HostMesh &m, const DevFieldsMixed &f,
     v.v = f.v;
     v.eos_order = f.eos_order;
     v.eos_rep = f.eos_rep;
+    v.rep = f.rep;
     return upload_mesh(m, v, stream);
 }
 

@@ -137,8 +137,8 @@ def test_launch_schedule_does_not_change_results(lulesh_gpu):
     """The sweep's launch shape and the Q/EOS band interleave only move work
     between blocks: every choice gives the same bits."""
     runs = {}
-    for shape, bands in (((32, 8, 16, 2), 3), ((32, 8, 5, 2), 1), ((16, 16, 7, 2), 4),
-                         ((64, 4, 3, 1), 7), ((32, 16, 9, 2), 2)):
+    for shape, bands in (((32, 8, 16, 2), 0), ((32, 8, 16, 2), 3), ((32, 8, 5, 2), 1),
+                         ((16, 16, 7, 2), 4), ((64, 4, 3, 1), 7), ((32, 16, 9, 2), 2)):
         d = lulesh_gpu.Domain(26, ic_mode=2, num_reg=11, cost=2, max_cycles=60)
         d.set_sweep_shape(*shape)
         d.set_eos_bands(bands)
@@ -151,7 +151,7 @@ def test_launch_schedule_does_not_change_results(lulesh_gpu):
             assert np.array_equal(bits(r[f]), bits(ref[f])), (key, f)
     d = lulesh_gpu.Domain(5)
     with pytest.raises(lulesh_gpu.LuleshError):
-        d.set_eos_bands(0)
+        d.set_eos_bands(-1)
 
 
 def test_largest_accepted_domain(lulesh_gpu):
