@@ -330,29 +330,49 @@ def bench_learned_q(lulesh_amd, efactor: float = 1.0) -> dict:
     (lulesh-accelerate_amd/learned_q_net.npz, trained by learned_q.py on
     fine 120^3 runs of other energy factors) at 40^3, beside the plain
     limiter at 40^3, both measured against the 120^3 run of the same factor
-    restricted to 40^3 at t = 1e-2."""
+    restricted to 40^3 at t = 1e-2; and the plain limiter at 40^3 to 64^3,
+    each against the 120^3 run restricted to its own grid, from which the
+    plain resolution reaching the learned run's errors and its wall time are
+    interpolated between measured runs (learned_q.measured_equivalent)."""
     import learned_q
     net, meta = learned_q.load()
-    res = learned_q.evaluate(net, [efactor])[f"{efactor:.4f}"]
+    res = learned_q.evaluate(net, [efactor], mid_ratio=0, form=meta["form"])[f"{efactor:.4f}"]
     plain, learned = res["plain"], res["learned"]
-    eq = res["learned_equivalent_plain"]
-    cost = [eq[k]["cost_ratio"] for k in learned_q.METRICS]
+    eq = res["learned_measured_equivalent"]
+    # the stand-in form (the network in the limiter's place) against the same
+    # plain runs
+    snet, smeta = learned_q.load(learned_q.NET_FILE_STANDIN)
+    sres = learned_q.evaluate(snet, [efactor], mid_ratio=0, sweep=(),
+                              form=smeta["form"])[f"{efactor:.4f}"]
+    standin = sres["learned"]
+    seq = learned_q.measured_equivalent(standin, standin["wall_s"], res["plain_sweep"])
+    speed = [eq[k]["wall_speedup"] for k in learned_q.METRICS]
+    keep = ("density_l1", "energy_l1", "pressure_l1", "cycles", "wall_s", "zone_cycles")
     return {
         "value": learned["zones_cycles_per_s"], "unit": "zones·cycles/s",
         "workload": "sedov 40^3 (LULESH 2.0 ICs, 11 regions) to t=1e-2, learned "
                     "closure vs plain limiter, errors against the 120^3 run "
-                    "restricted to 40^3",
-        "plain_40": plain, "learned_40": learned, "plain_80": res["plain_80"],
+                    "restricted to each grid",
+        "plain_40": plain, "learned_40": learned,
         "fine_120_cycles": res["fine_cycles"],
-        # the plain resolution whose error the learned 40^3 run matches (a
-        # power law through the plain errors at 40^3 and 80^3) and the
-        # zone*cycles that plain run would cost relative to 40^3
-        "learned_equivalent_plain": eq,
-        "equivalent_cost_ratio_geomean": float(np.exp(np.mean(np.log(cost)))),
+        "plain_sweep": [{k: r[k] for k in ("nx",) + keep} for r in res["plain_sweep"]],
+        # per metric: the plain resolution of equal error (interpolated
+        # between the two measured runs that bracket it) and that run's wall
+        # time and zone*cycles over the learned 40^3 run's
+        "learned_measured_equivalent": eq,
+        "wall_speedup_at_equal_error": {k: eq[k]["wall_speedup"] for k in learned_q.METRICS},
+        "wall_speedup_geomean": float(np.exp(np.mean(np.log(speed)))),
         "error_ratio_learned_over_plain": {
-            k: learned[k] / plain[k] for k in ("density_l1", "energy_l1", "pressure_l1")},
+            k: learned[k] / plain[k] for k in learned_q.METRICS},
         "training": {k: meta[k] for k in ("samples", "epochs", "loss_init", "loss_final",
                                           "loss_of_plain_limiter")},
+        "standin_40": {
+            "errors": {k: standin[k] for k in learned_q.METRICS},
+            "wall_speedup_at_equal_error": {k: seq[k]["wall_speedup"]
+                                            for k in learned_q.METRICS},
+            "plain_nx_at_equal_error": {k: seq[k]["plain_nx"] for k in learned_q.METRICS},
+            "zones_cycles_per_s": standin["zones_cycles_per_s"],
+        },
     }
 
 

@@ -363,8 +363,23 @@ const char *lulesh_mlp_last_error(const lulesh_mlp *n);
  * network until it is reset: the network must outlive that use.  Float64
  * domains only. */
 int lulesh_set_learned_q(lulesh_domain *d, const lulesh_mlp *net, double z_max);
+/* The form of the learned closure (no counterpart in the reference); it
+ * sets which inputs lulesh_q_features forms and what the network's outputs
+ * mean in the Q phase.  LULESH_LEARNED_FACTORS (the default): inputs include
+ * the limiter values phi, and the outputs multiply the limiter's ql and qq.
+ * LULESH_LEARNED_STANDIN: the network takes the limiter's place; its inputs
+ * are the limiter's inputs (the face velocity gradients relative to the
+ * element's own, velocity jumps over c, vdov, relative volume, sound speed)
+ * and its outputs are the limiting factors themselves,
+ * ql = -qlc rho m_l sum(du), qq = qqc rho m_q sum(du^2); the limiter
+ * (reference/C/LULESH-OMP.cc:2183-2320) runs only where an element falls
+ * back to it.  A network is trained for one form. */
+#define LULESH_LEARNED_FACTORS 0
+#define LULESH_LEARNED_STANDIN 1
+int lulesh_set_learned_q_form(lulesh_domain *d, int form);
 /* The network inputs of every element of the current state (features
- * [num_elem][12], in the order of lulesh_kernels.hip) after forming its
+ * [num_elem][12] of the form set by lulesh_set_learned_q_form, in the order
+ * of lulesh_kernels.hip) after forming its
  * kinematics and Q gradients, and q[num_elem][4] = (-rho sum du,
  * rho sum du^2, plain ql, plain qq), the first two 0 for an element without
  * compression.  Overwrites vnew, delv, arealg, vdov and the Q gradients;

@@ -110,6 +110,7 @@ struct lulesh_domain {
     int eos_bands = kDefaultEosBands;   // EOS schedule interleave (eos_chunk)
     const MlpFrags *learned = nullptr;  // learned-Q network, if set
     float learned_z_max = 0.f;
+    int learned_form = 0;               // LULESH_LEARNED_FACTORS / _STANDIN
     DevFields f{};
     DevFieldsMixed fm{};
     DevScalars *s = nullptr;
@@ -267,7 +268,7 @@ int queue_cycle(lulesh_domain *d)
                                     p, d->eos_bands);
              } else if (d->learned) {
                  hipLaunchKernelGGL(k_q_eos_dt_learned, dim3(ge), dim3(kBlock), 0, st, f, s,
-                                    p, d->learned, d->learned_z_max);
+                                    p, d->learned, d->learned_z_max, d->learned_form);
              } else if (d->eos_bands == 0) {
                  hipLaunchKernelGGL(k_q_eos_dt_nat, dim3(ge), dim3(kBlock), 0, st, f, s, p);
              } else {
@@ -1075,6 +1076,15 @@ int lulesh_set_learned_q(lulesh_domain *d, const lulesh_mlp *net, double z_max)
     return LULESH_OK;
 }
 
+int lulesh_set_learned_q_form(lulesh_domain *d, int form)
+{
+    if (!d || (form != LULESH_LEARNED_FACTORS && form != LULESH_LEARNED_STANDIN)) {
+        return LULESH_BAD_ARG;
+    }
+    d->learned_form = form;
+    return LULESH_OK;
+}
+
 int lulesh_q_features(lulesh_domain *d, float *features, double *q)
 {
     if (!d || !features || !q) {
@@ -1106,7 +1116,7 @@ int lulesh_q_features(lulesh_domain *d, float *features, double *q)
         return fail(d, LULESH_HIP_ERROR, "hipMalloc failed");
     }
     hipLaunchKernelGGL(k_q_features, dim3(ge), dim3(kBlock), 0, d->stream, d->f, d->prm,
-                       dfeat, dq);
+                       d->learned_form, dfeat, dq);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)
         e = hipMemcpyAsync(features, dfeat, ne * kMlpIn * sizeof(float),

@@ -416,6 +416,9 @@ namespace {
 // jumps across it (delv * delx, clamped to <= 0) and its density.
 struct QInputs {
     double phi[3];
+    // the face values the limiter of each direction compares with the
+    // element's own (low side, high side), boundary rules applied
+    double face[3][2];
     double du[3];
     double rho;
     double vdov, vnew, volo;
@@ -444,9 +447,15 @@ __device__ __forceinline__ void q_inputs(const DevFields &f, const PhysParams &p
     const double zp = f.delv_zeta[(k == nx - 1) ? el : el + sz];
     q_inputs_rest(f, el, dvxi, dveta, dvzeta, in);
     const double lm = prm.monoq_limiter_mult, ms = prm.monoq_max_slope;
-    in.phi[0] = mono_phi(dvxi, (i == 0) ? dvxi : xm, (i == nx - 1) ? 0.0 : xp, lm, ms);
-    in.phi[1] = mono_phi(dveta, (j == 0) ? dveta : ym, (j == nx - 1) ? 0.0 : yp, lm, ms);
-    in.phi[2] = mono_phi(dvzeta, (k == 0) ? dvzeta : zm, (k == nx - 1) ? 0.0 : zp, lm, ms);
+    in.face[0][0] = (i == 0) ? dvxi : xm;
+    in.face[0][1] = (i == nx - 1) ? 0.0 : xp;
+    in.face[1][0] = (j == 0) ? dveta : ym;
+    in.face[1][1] = (j == nx - 1) ? 0.0 : yp;
+    in.face[2][0] = (k == 0) ? dvzeta : zm;
+    in.face[2][1] = (k == nx - 1) ? 0.0 : zp;
+    in.phi[0] = mono_phi(dvxi, in.face[0][0], in.face[0][1], lm, ms);
+    in.phi[1] = mono_phi(dveta, in.face[1][0], in.face[1][1], lm, ms);
+    in.phi[2] = mono_phi(dvzeta, in.face[2][0], in.face[2][1], lm, ms);
 }
 
 // the neighbour value across one face of an unstructured element: own value
@@ -872,8 +881,10 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_batch(const BatchMember *__res
 }
 
 // ---------------------------------------------------------------------------
-// The learned closure.  The network sees twelve dimensionless inputs of an
-// element, formed from what the monotonic limiter works from:
+// The learned closure, in one of two forms (LearnedForm).
+//
+// Factors (form 0): the network sees twelve dimensionless inputs formed
+// from what the monotonic limiter works from and its result:
 //   0-2   the limiter values phi of the xi, eta, zeta directions
 //   3-5   the velocity jumps du (delv * delx, <= 0) over the sound speed c,
 //         as du / (c + |du|)
@@ -882,27 +893,57 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_batch(const BatchMember *__res
 //   10    log of the relative volume vnew (the density is m / (V0 vnew))
 //   11    log of the sound speed c of the last EOS evaluation
 // and returns two factors on the monotonic limiter's linear and quadratic
-// Q of the element, ql = m_l ql_limiter, qq = m_q qq_limiter, each clamped
-// to [0, 3] (m = 1 is the plain limiter).  The element keeps the plain
-// limiter when an input lies more than z_max standard deviations from the
-// training mean, when its volume or energy is not positive, or when it has
-// no compression to limit.
+// Q of the element, ql = m_l ql_limiter, qq = m_q qq_limiter.
+//
+// Stand-in (form 1): the network takes the limiter's place.  It sees the
+// limiter's inputs rather than its result:
+//   0-5   the velocity gradients across the element's faces relative to its
+//         own, r = delv_face / delv (low and high face of xi, eta, zeta; the
+//         boundary rules of the limiter applied), as r / (1 + |r|)
+//   6-8   the velocity jumps du / (c + |du|)
+//   9     vdov * hbar / (c + |vdov| hbar)
+//   10    log of the relative volume vnew (the density is m / (V0 vnew))
+//   11    log of the sound speed c
+// and its two outputs are the linear and quadratic limiting factors
+// themselves: ql = -qlc rho m_l sum du, qq = qqc rho m_q sum du^2 (the
+// plain limiter's factors are sum du (1 - phi) / sum du and
+// sum du^2 (1 - phi^2) / sum du^2); the limiter is evaluated only for the
+// elements that fall back to it.
+//
+// In both forms each output is clamped to [0, 3], and an element keeps the
+// plain limiter when an input lies more than z_max standard deviations from
+// the training mean, when its volume or energy is not positive, or when it
+// has no compression to limit.
 // ---------------------------------------------------------------------------
 namespace {
 
 __device__ __forceinline__ void learned_inputs(const DevFields &f, const QInputs &in,
-                                               int el, float feat[kMlpIn],
+                                               int el, int form, float feat[kMlpIn],
                                                double &ql_unlim, double &qq_unlim,
                                                bool &usable)
 {
     const double c = f.ss[el];
     const double hbar = (f.delx_xi[el] + f.delx_eta[el] + f.delx_zeta[el]) * (1.0 / 3.0);
     const double delx[3] = {f.delx_xi[el], f.delx_eta[el], f.delx_zeta[el]};
+    const double self[3] = {f.delv_xi[el], f.delv_eta[el], f.delv_zeta[el]};
+    if (form == 0) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        feat[d] = float(in.phi[d]);
-        feat[3 + d] = float(in.du[d] / (c + fabs(in.du[d]) + 1e-300));
-        feat[6 + d] = float(delx[d] / hbar);
+        for (int d = 0; d < 3; ++d) {
+            feat[d] = float(in.phi[d]);
+            feat[3 + d] = float(in.du[d] / (c + fabs(in.du[d]) + 1e-300));
+            feat[6 + d] = float(delx[d] / hbar);
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double norm = 1.0 / (self[d] + 1.e-36);
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                const double r = in.face[d][side] * norm;
+                feat[2 * d + side] = float(r / (1.0 + fabs(r)));
+            }
+            feat[6 + d] = float(in.du[d] / (c + fabs(in.du[d]) + 1e-300));
+        }
     }
     feat[9] = float(in.vdov * hbar / (c + fabs(in.vdov) * hbar + 1e-300));
     feat[10] = float(log(in.vnew > 0.0 ? in.vnew : 1.0));
@@ -920,7 +961,7 @@ __device__ __forceinline__ void learned_inputs(const DevFields &f, const QInputs
 // features of every element for training: feat[el][12] and
 // q[el][4] = (unlimited ql / qlc, unlimited qq / qqc, plain ql, plain qq);
 // the unlimited values are 0 where the element has no compression
-__global__ __launch_bounds__(256) void k_q_features(DevFields f, PhysParams prm,
+__global__ __launch_bounds__(256) void k_q_features(DevFields f, PhysParams prm, int form,
                                                     float *feat, double *q)
 {
     const int el = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
@@ -936,7 +977,7 @@ __global__ __launch_bounds__(256) void k_q_features(DevFields f, PhysParams prm,
     float x[kMlpIn];
     double ql_u, qq_u;
     bool usable;
-    learned_inputs(f, in, el, x, ql_u, qq_u, usable);
+    learned_inputs(f, in, el, form, x, ql_u, qq_u, usable);
     for (int t = 0; t < kMlpIn; ++t) {
         feat[size_t(el) * kMlpIn + t] = x[t];
     }
@@ -951,7 +992,7 @@ __global__ __launch_bounds__(256) void k_q_features(DevFields f, PhysParams prm,
 __global__ __launch_bounds__(256) void k_q_eos_dt_learned(DevFields f, DevScalars *s,
                                                           PhysParams prm,
                                                           const MlpFrags *net,
-                                                          float z_max)
+                                                          float z_max, int form)
 {
     __shared__ unsigned long long lds_c[4], lds_h[4];
     __shared__ int skip;
@@ -984,8 +1025,10 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_learned(DevFields f, DevScalar
         int i, j, k;
         elem_ijk(f, el, i, j, k);
         q_inputs(f, prm, el, i, j, k, in);
-        monotonic_q(prm, in, qlin, qquad);
-        learned_inputs(f, in, el, x, ql_u, qq_u, use_net);
+        if (form == 0) {
+            monotonic_q(prm, in, qlin, qquad);
+        }
+        learned_inputs(f, in, el, form, x, ql_u, qq_u, use_net);
     }
     // standardise; out-of-range inputs keep the plain limiter
 #pragma unroll
@@ -1005,8 +1048,16 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_learned(DevFields f, DevScalar
     if (use_net) {
         const float ml = fminf(fmaxf(L.y[lane][0], 0.f), 3.f);
         const float mq = fminf(fmaxf(L.y[lane][1], 0.f), 3.f);
-        qlin = double(ml) * qlin;
-        qquad = double(mq) * qquad;
+        if (form == 0) {
+            qlin = double(ml) * qlin;
+            qquad = double(mq) * qquad;
+        } else {
+            // the network's limiting factors on the unlimited sums
+            qlin = prm.qlc_monoq * (double(ml) * ql_u);
+            qquad = prm.qqc_monoq * (double(mq) * qq_u);
+        }
+    } else if (valid && form != 0) {
+        monotonic_q(prm, in, qlin, qquad);  // the fallback: the plain limiter
     }
     unsigned long long cand_c = as_bits(kBig), cand_h = as_bits(kBig);
     if (valid) {

@@ -238,9 +238,9 @@ __global__ void k_q_eos_dt_u(DevFields f, DevMeshU m, DevScalars *s, PhysParams 
 
 // The learned closure (network in lulesh_mlp.hpp).
 struct MlpFrags;
-__global__ void k_q_features(DevFields f, PhysParams prm, float *feat, double *q);
+__global__ void k_q_features(DevFields f, PhysParams prm, int form, float *feat, double *q);
 __global__ void k_q_eos_dt_learned(DevFields f, DevScalars *s, PhysParams prm,
-                                   const MlpFrags *net, float z_max);
+                                   const MlpFrags *net, float z_max, int form);
 
 // The mixed-precision phases (the time step kernels are shared).
 __global__ void k_kinematics_q_mixed(DevFieldsMixed f, DevScalars *s, PhysParams prm);

@@ -37,7 +37,8 @@ restricted fine one: L1 errors of density, energy and pressure over the
 coarse elements, relative to the fine solution's own L1 size, and the
 shock position.
 
-    python learned_q.py train [--out FILE] [--runs 16] [--fine 120]
+    python learned_q.py train [--out FILE] [--out-standin FILE] [--runs 16]
+                              [--fine 120]
     python learned_q.py evaluate [--net FILE] [--efactor 1.0 ...]
 """
 from __future__ import annotations
@@ -95,6 +96,26 @@ def restrict(fine: "lulesh_amd.Domain", r: int) -> dict:
     return out
 
 
+def restrict_to(fine: "lulesh_amd.Domain", nc: int) -> dict:
+    """The fine solution's v, e and p on an nc^3 grid of any size, by
+    material: the mesh is Lagrangian, so an element's logical index labels
+    its material, and coarse element (I, J, K) takes the fine elements whose
+    initial centroid lies in it, I = floor((i + 1/2) nc / nf).  v is the
+    mean relative volume (the aggregate's density is 1 / v, the initial
+    density being 1), e the mean energy (equal masses), p the volume-weighted
+    pressure.  For nc dividing nf this is the block restriction of
+    `restrict`."""
+    nf = fine.nx
+    m = ((np.arange(nf) + 0.5) * nc / nf).astype(np.int64)
+    cid = ((m[:, None, None] * nc + m[None, :, None]) * nc + m[None, None, :]).reshape(-1)
+    cnt = np.bincount(cid, minlength=nc ** 3).astype(np.float64)
+    v = fine.get("v")
+    sv = np.bincount(cid, weights=v, minlength=nc ** 3)
+    return {"v": sv / cnt,
+            "e": np.bincount(cid, weights=fine.get("e"), minlength=nc ** 3) / cnt,
+            "p": np.bincount(cid, weights=fine.get("p") * v, minlength=nc ** 3) / sv}
+
+
 def load_coarse(coarse: "lulesh_amd.Domain", st: dict) -> None:
     coarse.upload_state(st)
 
@@ -115,12 +136,24 @@ def smooth3(a: np.ndarray, n: int) -> np.ndarray:
     return (s / 27.0).reshape(-1)
 
 
+FORMS = (0, 1)  # lulesh_amd.Domain.LEARNED_FACTORS, LEARNED_STANDIN
+
+
 def search_targets(coarse, st0: dict, st1: dict, tol: float = 0.02):
-    """(inputs, targets, plain factors) of the compressing elements of the
-    restricted state st0, targets from the scale search over the interval
-    to st1 (see module text)."""
+    """Per form, (inputs, targets, plain-limiter outputs) of the compressing
+    elements of the restricted state st0, targets from the scale search over
+    the interval to st1 (see module text).  Factors form: the targets are the
+    best scales (c_l, c_q), the plain limiter (1, 1).  Stand-in form: the
+    targets are the limiting factors that give the same Q,
+    c_l * ql_plain / (qlc * ql_unlimited) and the same for qq, the plain
+    limiter's own factors the plain outputs."""
     load_coarse(coarse, st0)
-    feat, q = coarse.q_features()
+    feats = {}
+    for form in FORMS:
+        coarse.set_learned_q_form(form)
+        feats[form], q = coarse.q_features()
+    coarse.set_learned_q_form(FORMS[0])
+    feat = feats[FORMS[0]]
     keep = (q[:, 0] > 0) & ((q[:, 2] > 0) | (q[:, 3] > 0))
     if not keep.any():
         return None
@@ -153,9 +186,16 @@ def search_targets(coarse, st0: dict, st1: dict, tol: float = 0.02):
     gain = errs[i_plain] - errs[best, np.arange(n ** 3)]
     best = np.where(gain > tol * (errs[i_plain] + 1e-12), best, i_plain)
     t = np.asarray(pairs)[best]
-    ok = keep & np.all(np.isfinite(feat), axis=1)
-    return (feat[ok].astype(np.float32), t[ok].astype(np.float32),
-            np.ones((int(ok.sum()), 2), np.float32))
+    ok = keep & np.all(np.isfinite(feats[0]), axis=1) & np.all(np.isfinite(feats[1]), axis=1)
+    ok &= q[:, 1] > 0
+    out = {0: (feats[0][ok].astype(np.float32), t[ok].astype(np.float32),
+               np.ones((int(ok.sum()), 2), np.float32))}
+    plain_l = q[ok, 2] / (QLC * q[ok, 0])
+    plain_q = q[ok, 3] / (QQC * q[ok, 1])
+    out[1] = (feats[1][ok].astype(np.float32),
+              np.stack([t[ok, 0] * plain_l, t[ok, 1] * plain_q], axis=1).astype(np.float32),
+              np.stack([plain_l, plain_q], axis=1).astype(np.float32))
+    return out
 
 
 def run_efactors(seed: int, n: int) -> np.ndarray:
@@ -164,8 +204,9 @@ def run_efactors(seed: int, n: int) -> np.ndarray:
     return ef
 
 
-def generate(efactors, fine_nx: int = 120, ratio: int = 3, every: int = 200):
-    xs, ts, ps = [], [], []
+def generate(efactors, fine_nx: int = 120, ratio: int = 3, every: int = 200) -> dict:
+    """Training data of both forms: form -> (inputs, targets, plain outputs)."""
+    data = {form: ([], [], []) for form in FORMS}
     coarse = lulesh_amd.Domain(fine_nx // ratio)
     for f in efactors:
         t0 = time.perf_counter()
@@ -180,9 +221,9 @@ def generate(efactors, fine_nx: int = 120, ratio: int = 3, every: int = 200):
             if prev is not None:
                 s = search_targets(coarse, prev, cur)
                 if s is not None:
-                    xs.append(s[0])
-                    ts.append(s[1])
-                    ps.append(s[2])
+                    for form in FORMS:
+                        for lst, a in zip(data[form], s[form]):
+                            lst.append(a)
                     nsnap += 1
             prev = cur
             if done < every:
@@ -190,15 +231,16 @@ def generate(efactors, fine_nx: int = 120, ratio: int = 3, every: int = 200):
         st = fine.stats()
         fine.close()
         log(f"learned_q: fine run efactor={f:.4f}: {st.cycle} cycles, {nsnap} "
-            f"snapshots, {sum(len(x) for x in xs)} samples so far, "
+            f"snapshots, {sum(len(x) for x in data[0][0])} samples so far, "
             f"{time.perf_counter() - t0:.1f} s")
     coarse.close()
-    return np.concatenate(xs), np.concatenate(ts), np.concatenate(ps)
+    return {form: tuple(np.concatenate(lst) for lst in data[form]) for form in FORMS}
 
 
 def init_params(seed: int) -> np.ndarray:
-    """He-scaled normal weights, zero hidden biases; the output starts
-    near the plain limiter's factors (1, 1)."""
+    """He-scaled normal weights, zero hidden biases, output biases 1 (the
+    plain limiter's factors in the factors form, full limiting factors in
+    the stand-in form)."""
     rng = np.random.default_rng(seed)
     w1 = rng.normal(0, np.sqrt(2 / 12), (12, 64))
     w2 = rng.normal(0, np.sqrt(2 / 64), (64, 64))
@@ -225,10 +267,16 @@ def save(path: str, net, meta: dict) -> None:
              meta=np.array(json.dumps(meta)))
 
 
+NET_FILE_STANDIN = os.path.join(PKG, "learned_q_standin_net.npz")
+
+
 def load(path: str = NET_FILE):
+    """(network, meta) of a saved network; meta["form"] is its form (0 for
+    files saved without one)."""
     with np.load(path, allow_pickle=False) as z:
         net = lulesh_amd.Mlp(z["params"], z["in_mean"], z["in_std"])
         meta = json.loads(str(z["meta"]))
+    meta.setdefault("form", 0)
     return net, meta
 
 
@@ -270,11 +318,57 @@ def equivalent_resolution(nc: int, plain: dict, mid: int, plain_mid: dict,
     return out
 
 
-def evaluate(net, efactors, fine_nx=120, ratio=3, z_max=4.0, mid_ratio=2) -> dict:
+def measured_equivalent(learned: dict, learned_wall: float, sweep: list[dict]) -> dict:
+    """Per metric, the plain resolution whose measured error equals the
+    learned run's, interpolated (log-log) between the two measured plain
+    resolutions that bracket it, and the plain run's wall time and
+    zone*cycles there (interpolated the same way between the two measured
+    runs) over the learned run's: the speed-up at equal error.  When the
+    learned error lies beyond the measured range the nearest run bounds it
+    and "bound" says which way."""
+    sweep = sorted(sweep, key=lambda r: r["nx"])
+    out = {}
+    for k in METRICS:
+        target = learned[k]
+        errs = [r[k] for r in sweep]
+        row = None
+        for a, b in zip(sweep, sweep[1:]):
+            ea, eb = a[k], b[k]
+            if (ea - target) * (eb - target) <= 0 and ea != eb:
+                w = np.log(target / ea) / np.log(eb / ea)
+
+                def interp(key):
+                    return float(np.exp(np.log(a[key]) + w * (np.log(b[key]) - np.log(a[key]))))
+
+                n_eq = float(np.exp(np.log(a["nx"]) + w * (np.log(b["nx"]) - np.log(a["nx"]))))
+                row = {"plain_nx": n_eq, "bracket": [a["nx"], b["nx"]], "bound": "measured",
+                       "plain_wall_s": interp("wall_s"),
+                       "plain_zone_cycles": interp("zone_cycles")}
+                break
+        if row is None:
+            last = sweep[-1] if target < min(errs) else sweep[0]
+            row = {"plain_nx": float(last["nx"]), "bracket": [last["nx"], last["nx"]],
+                   "bound": "above" if target < min(errs) else "below",
+                   "plain_wall_s": last["wall_s"], "plain_zone_cycles": last["zone_cycles"]}
+        row["wall_speedup"] = row["plain_wall_s"] / learned_wall
+        row["zone_cycle_ratio"] = row["plain_zone_cycles"] / learned["zone_cycles"]
+        out[k] = row
+    return out
+
+
+PLAIN_SWEEP = (40, 44, 48, 52, 56, 60, 64)
+
+
+def evaluate(net, efactors, fine_nx=120, ratio=3, z_max=4.0, mid_ratio=2,
+             sweep=PLAIN_SWEEP, form: int = 0) -> dict:
     """Errors of the plain and learned limiters at fine_nx / ratio against
     the fine run restricted to that grid, and of the plain limiter at
     mid_ratio times that resolution (restricted the same way), from which
-    the plain resolution matching the learned errors is estimated."""
+    the plain resolution matching the learned errors is estimated by a power
+    law; and of the plain limiter at every resolution of `sweep`, each
+    against the fine run restricted to its own grid (restrict_to), from
+    which the plain resolution of equal error and its cost are measured
+    (measured_equivalent)."""
     out = {}
     nc = fine_nx // ratio
     for f in efactors:
@@ -282,20 +376,22 @@ def evaluate(net, efactors, fine_nx=120, ratio=3, z_max=4.0, mid_ratio=2) -> dic
         fine = lulesh_amd.Domain(fine_nx, efactor=float(f))
         fine.run()
         ref = restrict(fine, ratio)
+        refs = {n: restrict_to(fine, n) for n in sweep}
         fine_cycles = fine.stats().cycle
         fine.close()
         row = {"fine_cycles": fine_cycles}
         for name in ("plain", "learned"):
             d = lulesh_amd.Domain(nc, efactor=float(f))
             if name == "learned":
-                d.set_learned_q(net, z_max)
+                d.set_learned_q(net, z_max, form)
             d.step(1)
             t1 = time.perf_counter()
             rc, n = d.run()
             wall = time.perf_counter() - t1
             st = d.stats()
             row[name] = errors(d, ref)
-            row[name].update({"cycles": st.cycle, "rc": rc,
+            row[name].update({"cycles": st.cycle, "rc": rc, "wall_s": wall,
+                              "zone_cycles": nc ** 3 * n,
                               "zones_cycles_per_s": nc ** 3 * n / wall})
             d.close()
         if mid_ratio:
@@ -311,6 +407,22 @@ def evaluate(net, efactors, fine_nx=120, ratio=3, z_max=4.0, mid_ratio=2) -> dic
             d.close()
             row["learned_equivalent_plain"] = equivalent_resolution(
                 nc, row["plain"], nm, row[f"plain_{nm}"], row["learned"])
+        if sweep:
+            runs = []
+            for n_s in sweep:
+                d = lulesh_amd.Domain(n_s, efactor=float(f))
+                d.step(1)
+                t1 = time.perf_counter()
+                rc, n = d.run()
+                wall = time.perf_counter() - t1
+                r = errors(d, refs[n_s])
+                r.update({"nx": n_s, "cycles": d.stats().cycle, "rc": rc, "wall_s": wall,
+                          "zone_cycles": n_s ** 3 * n})
+                d.close()
+                runs.append(r)
+            row["plain_sweep"] = runs
+            row["learned_measured_equivalent"] = measured_equivalent(
+                row["learned"], row["learned"]["wall_s"], runs)
         out[f"{float(f):.4f}"] = row
         log(f"learned_q: efactor {f:.4f}: plain {row['plain']['density_l1']:.4f} "
             f"learned {row['learned']['density_l1']:.4f} (density L1), "
@@ -322,6 +434,7 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("mode", choices=("train", "evaluate"))
     ap.add_argument("--out", default=NET_FILE)
+    ap.add_argument("--out-standin", default=NET_FILE_STANDIN)
     ap.add_argument("--net", default=NET_FILE)
     ap.add_argument("--runs", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1)
@@ -335,20 +448,25 @@ def main() -> int:
     args = ap.parse_args()
     if args.mode == "train":
         ef = run_efactors(args.seed, args.runs)
-        x, t, plain = generate(ef, args.fine, args.ratio, args.every)
-        net, loss0, loss = train(x, t, epochs=args.epochs, lr=args.lr, seed=args.seed)
-        plain_loss = float(0.5 * np.mean(np.sum((plain - t) ** 2, axis=1)))
-        meta = {"fine": args.fine, "ratio": args.ratio, "every": args.every,
-                "criterion": "density+energy+pressure",
-                "efactors": [float(v) for v in ef], "samples": int(len(x)),
-                "epochs": args.epochs, "lr": args.lr, "loss_init": loss0,
-                "loss_final": loss, "loss_of_plain_limiter": plain_loss}
-        save(args.out, net, meta)
-        log(f"learned_q: saved {args.out}: {json.dumps(meta)}")
-        print(json.dumps(meta))
+        data = generate(ef, args.fine, args.ratio, args.every)
+        metas = {}
+        for form, path in ((0, args.out), (1, args.out_standin)):
+            x, t, plain = data[form]
+            net, loss0, loss = train(x, t, epochs=args.epochs, lr=args.lr, seed=args.seed)
+            plain_loss = float(0.5 * np.mean(np.sum((plain - t) ** 2, axis=1)))
+            meta = {"form": form, "fine": args.fine, "ratio": args.ratio,
+                    "every": args.every, "criterion": "density+energy+pressure",
+                    "efactors": [float(v) for v in ef], "samples": int(len(x)),
+                    "epochs": args.epochs, "lr": args.lr, "loss_init": loss0,
+                    "loss_final": loss, "loss_of_plain_limiter": plain_loss}
+            save(path, net, meta)
+            log(f"learned_q: saved {path}: {json.dumps(meta)}")
+            metas[form] = meta
+        print(json.dumps(metas))
     else:
         net, meta = load(args.net)
-        res = evaluate(net, args.efactor, args.fine, args.ratio, args.z_max)
+        res = evaluate(net, args.efactor, args.fine, args.ratio, args.z_max,
+                       form=meta["form"])
         print(json.dumps({"net": meta, "evaluation": res}))
     return 0
 

@@ -79,7 +79,8 @@ EXPORTS = (
     "lulesh_mlp_get_params", "lulesh_mlp_eval", "lulesh_mlp_gradient",
     "lulesh_mlp_gradient_noalloc",
     "lulesh_mlp_train", "lulesh_mlp_last_error", "lulesh_set_learned_q",
-    "lulesh_q_features", "lulesh_write_domain_arrays", "lulesh_write_domain",
+    "lulesh_q_features", "lulesh_set_learned_q_form", "lulesh_write_domain_arrays",
+    "lulesh_write_domain",
     "lulesh_create_lmesh", "lulesh_create_mesh", "lulesh_origin_element",
     "lulesh_get_nodelist", "lulesh_set_eos_bands",
 )
@@ -213,6 +214,8 @@ def lib() -> ctypes.CDLL:
     L.lulesh_set_learned_q.restype = I
     L.lulesh_q_features.argtypes = [P, FP, DP]
     L.lulesh_q_features.restype = I
+    L.lulesh_set_learned_q_form.argtypes = [P, I]
+    L.lulesh_set_learned_q_form.restype = I
     L.lulesh_write_domain_arrays.argtypes = [ctypes.c_char_p, I, I, D] + [DP] * 10
     L.lulesh_write_domain_arrays.restype = I
     L.lulesh_write_domain.argtypes = [P, ctypes.c_char_p]
@@ -429,12 +432,21 @@ class Domain:
             float(st["deltatime"])))
 
     # -- learned closure ------------------------------------------------
-    def set_learned_q(self, net, z_max: float = 4.0) -> None:
-        """Uses the network `net` (an Mlp, kept alive by this domain) in
-        place of the monotonic limiter; None restores the limiter."""
+    LEARNED_FACTORS, LEARNED_STANDIN = 0, 1
+
+    def set_learned_q(self, net, z_max: float = 4.0, form: int | None = None) -> None:
+        """Uses the network `net` (an Mlp, kept alive by this domain) in the
+        Q phase, in the given form (LEARNED_FACTORS: factors on the
+        limiter's Q; LEARNED_STANDIN: in the limiter's place; None: keep
+        the form set); net None restores the limiter."""
+        if form is not None:
+            self.set_learned_q_form(form)
         self._net = net
         self._check(lib().lulesh_set_learned_q(self.h, net.h if net else None,
                                                float(z_max)))
+
+    def set_learned_q_form(self, form: int) -> None:
+        self._check(lib().lulesh_set_learned_q_form(self.h, int(form)))
 
     def q_features(self) -> tuple[np.ndarray, np.ndarray]:
         """(features[num_elem, 12], q[num_elem, 4]) of the current state."""
