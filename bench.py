@@ -4,8 +4,11 @@ A step is one LULESH cycle (TimeIncrement + LagrangeLeapFrog) of the
 benchmark workload: the Sedov blast problem at 150^3 elements (3.375 M
 zones, 151^3 nodes) with the LULESH 2.0 multi-region layout (11 regions,
 balance 1, cost 1, seeded region assignment) and LULESH 2.0 initial
-conditions, in float64.  The domain is built once and stays resident on the
-GPU; the timed region is K consecutive cycles after W warm-up cycles.
+conditions, in float64 with fused multiply-adds (results within the
+reference's tolerances; the bit-identical step, every value rounded as the
+reference rounds it, is timed the same way and reported beside it).  The
+domain is built once and stays resident on the GPU; the timed region is K
+consecutive cycles after W warm-up cycles.
 
 Prints one JSON line: zones*cycles/s of the timed cycles (queued back to
 back, no events between kernels), the roofline of the dominant kernel
@@ -376,6 +379,49 @@ def bench_learned_q(lulesh_amd, efactor: float = 1.0) -> dict:
     }
 
 
+def time_plain(lulesh_amd, nx: int, K: int, W: int, fused: bool) -> dict | None:
+    """The plain float64 step of the benchmark workload: KR cycles that take
+    the GPU from idle to its sustained clock, the per-kernel breakdown (HIP
+    events around every launch on the domain's stream) over KT cycles, W
+    warm-up cycles, then K timed cycles queued back to back with no events
+    between the kernels.  From idle, the first 40 cycles of this domain run
+    up to 10% slower (the sweep 346 us in cycles 1-10, 291 us from cycle 40
+    on); a GPU kept busy beforehand runs cycles 6-25 at the steady-state
+    rate.  The events cost about 1% of a cycle, so they stay out of the
+    timed cycles."""
+    KR, KT = 25, 25
+    d = lulesh_amd.Domain(nx, ic_mode=2, num_reg=11, balance=1, cost=1,
+                          max_cycles=KR + KT + W + K, fused=fused)
+    rc, done = d.step(KR)
+    if rc != 0 or done != KR:
+        print(f"bench: step returned rc={rc}, {done} of {KR} cycles", file=sys.stderr)
+        return None
+    d.set_kernel_timing(True)
+    rc, done = d.step(KT)
+    if rc != 0 or done != KT:
+        print(f"bench: breakdown step returned rc={rc}, {done} of {KT} cycles",
+              file=sys.stderr)
+        return None
+    times = d.kernel_times()
+    d.set_kernel_timing(False)
+    if W:
+        d.step(W)
+    # the timed region: lulesh_step is synchronous, so the device is idle
+    # before and after
+    t0 = time.perf_counter()
+    rc, done = d.step(K)
+    wall = time.perf_counter() - t0
+    if rc != 0 or done != K:
+        print(f"bench: step returned rc={rc}, {done} of {K} cycles", file=sys.stderr)
+        return None
+    st = d.stats()
+    d.close()
+    print(f"bench: {nx}^3 {'fused' if fused else 'exact'}: {wall * 1e3 / K:.4f} ms/cycle",
+          file=sys.stderr, flush=True)
+    return {"wall": wall, "times": times, "stats": st, "kt": KT,
+            "first_timed": KR + KT + W + 1, "breakdown": [KR + 1, KR + KT]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -398,44 +444,19 @@ def main() -> int:
     import lulesh_amd
 
     nx, K, W = args.nx, args.steps, args.warmup
-    # Before the warm-up: KR cycles that take the GPU from idle to its
-    # sustained clock, then the per-kernel breakdown (HIP events around
-    # every launch on the domain's stream) over KT cycles.  From idle, the
-    # first 40 cycles of this domain run up to 10% slower (the sweep 346 us
-    # in cycles 1-10, 291 us from cycle 40 on); a GPU kept busy beforehand
-    # runs cycles 6-25 at the steady-state rate.  The events cost about 1%
-    # of a cycle, so they stay out of the timed cycles.
-    KR, KT = 25, 25
-    d = lulesh_amd.Domain(nx, ic_mode=2, num_reg=11, balance=1, cost=1,
-                          max_cycles=KR + KT + W + K)
-    rc, done = d.step(KR)
-    if rc != 0 or done != KR:
-        print(f"bench: step returned rc={rc}, {done} of {KR} cycles", file=sys.stderr)
-        return 1
-    d.set_kernel_timing(True)
-    rc, done = d.step(KT)
-    if rc != 0 or done != KT:
-        print(f"bench: breakdown step returned rc={rc}, {done} of {KT} cycles",
-              file=sys.stderr)
-        return 1
-    times = d.kernel_times()
-    d.set_kernel_timing(False)
-    if W:
-        d.step(W)
-    # the timed region: K cycles queued back to back, no events between the
-    # kernels; lulesh_step is synchronous, so the device is idle before and
-    # after
-    t0 = time.perf_counter()
-    rc, done = d.step(K)
-    wall = time.perf_counter() - t0
-    if rc != 0 or done != K:
-        print(f"bench: step returned rc={rc}, {done} of {K} cycles",
-              file=sys.stderr)
-        return 1
-    st = d.stats()
-    d.close()
-
     ne, nn = nx ** 3, (nx + 1) ** 3
+    # the float64 step with fused multiply-adds (results within the
+    # reference's tolerances, tests/test_gpu_fused.py), then the same with
+    # every value rounded as the reference rounds it (bit-identical)
+    runs = {}
+    for fused in (True, False):
+        r = time_plain(lulesh_amd, nx, K, W, fused)
+        if r is None:
+            return 1
+        runs[fused] = r
+    fz, ex = runs[True], runs[False]
+    wall, times, st = fz["wall"], fz["times"], fz["stats"]
+    KT = fz["kt"]
     zones_cycles = ne * K
     value = zones_cycles / wall
     # dominant kernel and its roofline
@@ -446,6 +467,7 @@ def main() -> int:
     achieved = alg_bytes / (per_kernel[dom] * 1e-3) / 1e9
     b_cycle = 520 * ne + 296 * nn
     roof_value = HBM_PEAK_GBS * 1e9 / b_cycle * ne
+    ex_value = ne * K / ex["wall"]
     result = {
         "metric": METRIC,
         "value": value,
@@ -462,7 +484,10 @@ def main() -> int:
         "config": {"workload": f"sedov_{nx}^3_lulesh2.0_regions11_balance1_cost1",
                    "nx": nx, "zones": ne, "nodes": nn, "ic": "LULESH 2.0.3",
                    "num_reg": 11, "balance": 1, "cost": 1,
-                   "cycles_timed": K, "first_timed_cycle": KR + KT + W + 1,
+                   "arithmetic": "float64 with fused multiply-adds "
+                                 "(lulesh_set_fused_multiply_add): within the "
+                                 "reference's tolerances, not bit-identical",
+                   "cycles_timed": K, "first_timed_cycle": fz["first_timed"],
                    "final_cycle": st.cycle},
         "roofline": {"bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -475,7 +500,15 @@ def main() -> int:
                            "frac": value / roof_value},
         "kernels_mean_ms": {k: round(v, 6) for k, v in per_kernel.items()},
         "gpu_event_ms_per_step": sum(ms for ms, _ in times.values()) / KT,
-        "breakdown_cycles": [KR + 1, KR + KT],
+        "breakdown_cycles": fz["breakdown"],
+        # the same workload with every value rounded as the reference rounds
+        # it: bit-identical to the reference (tests/test_gpu_parity.py)
+        "exact_rounding": {
+            "value": ex_value, "ms_per_step": ex["wall"] * 1e3 / K,
+            "cycle_roofline_frac": ex_value / roof_value,
+            "kernels_mean_ms": {k: round(ms / n, 6) for k, (ms, n) in ex["times"].items()
+                                if n},
+        },
     }
     forms = [] if args.no_forms else [f for f in args.forms.split(",") if f]
     unknown = set(forms) - {"batched", "mixed", "learned_q", "unstructured"}

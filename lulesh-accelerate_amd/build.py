@@ -5,7 +5,9 @@
 
 Both land next to this file so they travel with the source tree.  Device
 code is compiled without FMA contraction (the kernels reproduce the
-reference's rounding) and without packed-FP32 instructions.
+reference's rounding) and without packed-FP32 instructions; the element
+phases are compiled a second time with contraction (LULESH_FUSED, the
+fused-multiply-add mode of the float64 step).
 """
 from __future__ import annotations
 
@@ -27,6 +29,8 @@ HIP_SOURCES = ["lulesh_kernels.hip", "lulesh_force.hip", "lulesh_mixed.hip",
                "lulesh_mesh.cpp"]
 HEADERS = ["lulesh_kernels.hpp", "lulesh_elem.hpp", "lulesh_cbrt.h",
            "lulesh_mesh.hpp", "lulesh_setup.hpp", "lulesh_mlp.hpp"]
+# built a second time with multiply-adds fused (namespace lulesh::fused)
+FUSED_SOURCES = ["lulesh_force.hip", "lulesh_kernels.hip"]
 OBJ_DIR = os.path.join(PKG, "_build")
 
 COMMON_FLAGS = [
@@ -63,6 +67,12 @@ def build_library() -> str:
         lang = ["-x", "hip"] if src.endswith(".hip") else []
         jobs.append((obj, [cc] + COMMON_FLAGS + ["-fPIC"] + lang +
                      ["-c", os.path.join(CSRC, src), "-o", obj]))
+    fused_flags = [("-ffp-contract=fast" if f == "-ffp-contract=off" else f)
+                   for f in COMMON_FLAGS] + ["-DLULESH_FUSED"]
+    for src in FUSED_SOURCES:
+        obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + "_fused.o")
+        jobs.append((obj, [cc] + fused_flags + ["-fPIC", "-x", "hip", "-c",
+                                                os.path.join(CSRC, src), "-o", obj]))
     workers = max(1, min(len(jobs), os.cpu_count() or 1, 8))
     with ThreadPoolExecutor(max_workers=workers) as pool:
         list(pool.map(lambda j: _run(j[1]), jobs))

@@ -103,6 +103,7 @@ struct lulesh_domain {
     std::vector<int32_t> reg_num_list;
 
     bool mixed = false;   // float32 element-local arithmetic (fm), else f
+    bool fused = false;   // float64 phases with multiply-adds fused (lulesh::fused)
     bool unstructured = false;  // an explicit-connectivity mesh (mesh), f
     DevMeshU mesh{};
     int origin = 0;             // the element holding the initial energy
@@ -248,8 +249,12 @@ int queue_cycle(lulesh_domain *d)
              hipLaunchKernelGGL(k_time_increment, dim3(1), dim3(64), 0, st, s, p.ic_mode);
          })))
         return rc;
+    const bool fused = d->fused;
     if ((rc = launch(d, KID_FORCE_SWEEP, [&]() -> hipError_t {
-             return mixed ? launch_force_sweep_mixed(d->sweep, fm, s, p, st)
+             if (mixed) {
+                 return launch_force_sweep_mixed(d->sweep, fm, s, p, st);
+             }
+             return fused ? fused::launch_force_sweep(d->sweep, f, s, p, st)
                           : launch_force_sweep(d->sweep, f, s, p, st);
          })))
         return rc;
@@ -257,6 +262,9 @@ int queue_cycle(lulesh_domain *d)
              if (mixed) {
                  hipLaunchKernelGGL(k_kinematics_q_mixed, dim3(ge), dim3(kBlock), 0, st, fm,
                                     s, p);
+             } else if (fused) {
+                 hipLaunchKernelGGL(fused::k_kinematics_q, dim3(ge), dim3(kBlock), 0, st, f, s,
+                                    p);
              } else {
                  hipLaunchKernelGGL(k_kinematics_q, dim3(ge), dim3(kBlock), 0, st, f, s, p);
              }
@@ -267,13 +275,15 @@ int queue_cycle(lulesh_domain *d)
                  hipLaunchKernelGGL(k_q_eos_dt_mixed, dim3(ge), dim3(kBlock), 0, st, fm, s,
                                     p, d->eos_bands);
              } else if (d->learned) {
-                 hipLaunchKernelGGL(k_q_eos_dt_learned, dim3(ge), dim3(kBlock), 0, st, f, s,
-                                    p, d->learned, d->learned_z_max, d->learned_form);
+                 hipLaunchKernelGGL(fused ? fused::k_q_eos_dt_learned : k_q_eos_dt_learned,
+                                    dim3(ge), dim3(kBlock), 0, st, f, s, p, d->learned,
+                                    d->learned_z_max, d->learned_form);
              } else if (d->eos_bands == 0) {
-                 hipLaunchKernelGGL(k_q_eos_dt_nat, dim3(ge), dim3(kBlock), 0, st, f, s, p);
+                 hipLaunchKernelGGL(fused ? fused::k_q_eos_dt_nat : k_q_eos_dt_nat, dim3(ge),
+                                    dim3(kBlock), 0, st, f, s, p);
              } else {
-                 hipLaunchKernelGGL(k_q_eos_dt, dim3(ge), dim3(kBlock), 0, st, f, s, p,
-                                    d->eos_bands);
+                 hipLaunchKernelGGL(fused ? fused::k_q_eos_dt : k_q_eos_dt, dim3(ge),
+                                    dim3(kBlock), 0, st, f, s, p, d->eos_bands);
              }
          })))
         return rc;
@@ -1073,6 +1083,19 @@ int lulesh_set_learned_q(lulesh_domain *d, const lulesh_mlp *net, double z_max)
     }
     d->learned = net ? lulesh_mlp_device_frags(net) : nullptr;
     d->learned_z_max = float(z_max);
+    return LULESH_OK;
+}
+
+int lulesh_set_fused_multiply_add(lulesh_domain *d, int enable)
+{
+    if (!d) {
+        return LULESH_BAD_ARG;
+    }
+    if (d->mixed || d->unstructured) {
+        return fail(d, LULESH_BAD_ARG,
+                    "fused multiply-adds are a mode of the float64 structured step");
+    }
+    d->fused = enable != 0;
     return LULESH_OK;
 }
 

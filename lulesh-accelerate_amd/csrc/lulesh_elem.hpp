@@ -11,10 +11,11 @@
 
 #include <hip/hip_runtime.h>
 
-// The mixed-precision translation unit defines LULESH_MIXED_TU: its results
-// are not bit-matched, so there the element templates may fuse
-// multiply-adds.  Everywhere else contraction stays off.
-#ifdef LULESH_MIXED_TU
+// The mixed-precision translation unit defines LULESH_MIXED_TU and the
+// fused float64 build LULESH_FUSED: their results are not bit-matched, so
+// there the element templates may fuse multiply-adds.  Everywhere else
+// contraction stays off.
+#if defined(LULESH_MIXED_TU) || defined(LULESH_FUSED)
 #pragma clang fp contract(fast)
 #else
 #pragma clang fp contract(off)

@@ -28,9 +28,16 @@
 
 #include "lulesh_elem.hpp"
 
+// FMA contraction off: every value rounded as the reference rounds it;
+// the fused build (LULESH_FUSED) lets multiply-adds fuse
+#ifdef LULESH_FUSED
+#pragma clang fp contract(fast)
+#else
 #pragma clang fp contract(off)
+#endif
 
 namespace lulesh {
+LULESH_VARIANT_BEGIN
 
 namespace {
 
@@ -778,4 +785,5 @@ hipError_t launch_force_sweep_batch(const SweepShape &sh, int blocks,
     return hipErrorInvalidValue;
 }
 
+LULESH_VARIANT_END
 }  // namespace lulesh

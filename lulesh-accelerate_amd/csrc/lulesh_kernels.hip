@@ -25,9 +25,16 @@
 #include "lulesh_elem.hpp"
 #include "lulesh_mlp.hpp"
 
+// FMA contraction off: every value rounded as the reference rounds it;
+// the fused build (LULESH_FUSED) lets multiply-adds fuse
+#ifdef LULESH_FUSED
+#pragma clang fp contract(fast)
+#else
 #pragma clang fp contract(off)
+#endif
 
 namespace lulesh {
+LULESH_VARIANT_BEGIN
 
 namespace {
 
@@ -1172,4 +1179,5 @@ __global__ void k_device_math(int op, int n, const double *a, const double *b,
     }
 }
 
+LULESH_VARIANT_END
 }  // namespace lulesh

@@ -17,6 +17,19 @@
 
 #include <cstdint>
 
+// The element phases (lulesh_force.hip, lulesh_kernels.hip) are compiled
+// twice: with FMA contraction off, every value rounded as the reference
+// rounds it (namespace lulesh), and with LULESH_FUSED defined, multiply-adds
+// fused into one rounding (namespace lulesh::fused; selected per domain by
+// lulesh_set_fused_multiply_add).
+#ifdef LULESH_FUSED
+#define LULESH_VARIANT_BEGIN namespace fused {
+#define LULESH_VARIANT_END }
+#else
+#define LULESH_VARIANT_BEGIN
+#define LULESH_VARIANT_END
+#endif
+
 namespace lulesh {
 
 // Scalars of the cycle loop, resident in device memory so that TimeIncrement
@@ -249,5 +262,18 @@ hipError_t launch_force_sweep_mixed(const SweepShape &shape, const DevFieldsMixe
                                     DevScalars *s, const PhysParams &prm,
                                     hipStream_t stream);
 bool force_sweep_mixed_shape_supported(const SweepShape &shape);
+
+// The float64 phases with multiply-adds fused (the same sources built with
+// LULESH_FUSED).
+namespace fused {
+__global__ void k_kinematics_q(DevFields f, DevScalars *s, PhysParams prm);
+__global__ void k_q_eos_dt(DevFields f, DevScalars *s, PhysParams prm, int bands);
+__global__ void k_q_eos_dt_nat(DevFields f, DevScalars *s, PhysParams prm);
+__global__ void k_q_eos_dt_learned(DevFields f, DevScalars *s, PhysParams prm,
+                                   const MlpFrags *net, float z_max, int form);
+hipError_t launch_force_sweep(const SweepShape &shape, const DevFields &f,
+                              DevScalars *s, const PhysParams &prm,
+                              hipStream_t stream);
+}  // namespace fused
 
 }  // namespace lulesh

@@ -62,7 +62,8 @@ _lib = None
 EXPORTS = (
     "lulesh_config_default", "lulesh_create", "lulesh_destroy",
     "lulesh_create_mixed", "lulesh_create_scaled", "lulesh_is_mixed", "lulesh_step",
-    "lulesh_run", "lulesh_upload_state", "lulesh_download_state",
+    "lulesh_run", "lulesh_set_fused_multiply_add", "lulesh_upload_state",
+    "lulesh_download_state",
     "lulesh_get_field", "lulesh_set_field", "lulesh_get_stats",
     "lulesh_set_scalars", "lulesh_set_parameter", "lulesh_set_sweep_shape",
     "lulesh_get_sweep_shape", "lulesh_get_regions", "lulesh_symmetry",
@@ -127,6 +128,8 @@ def lib() -> ctypes.CDLL:
     L.lulesh_step.restype = I
     L.lulesh_run.argtypes = [P, IP]
     L.lulesh_run.restype = I
+    L.lulesh_set_fused_multiply_add.argtypes = [P, I]
+    L.lulesh_set_fused_multiply_add.restype = I
     L.lulesh_upload_state.argtypes = [P] + [DP] * 11 + [D, D]
     L.lulesh_upload_state.restype = I
     L.lulesh_download_state.argtypes = [P] + [DP] * 13
@@ -240,9 +243,11 @@ class Domain:
     def __init__(self, nx: int = 30, ic_mode: int = 2, num_reg: int = 11,
                  balance: int = 1, cost: int = 1, max_cycles: int = 9999999,
                  stoptime: float = 1.0e-2, mixed: bool = False,
-                 efactor: float = 1.0):
+                 efactor: float = 1.0, fused: bool = False):
         """mixed=True builds the mixed-precision form (lulesh_create_mixed);
-        efactor multiplies the initial origin energy."""
+        efactor multiplies the initial origin energy; fused=True runs the
+        float64 step with fused multiply-adds
+        (lulesh_set_fused_multiply_add)."""
         L = lib()
         cfg = default_config()
         cfg.nx, cfg.ic_mode, cfg.num_reg = nx, ic_mode, num_reg
@@ -261,8 +266,15 @@ class Domain:
         self.h = h
         self.nx, self.ic_mode = nx, ic_mode
         self.mixed = bool(L.lulesh_is_mixed(h))
+        self.fused = False
+        if fused:
+            self.set_fused_multiply_add(True)
         st = self.stats()
         self.num_elem, self.num_node = st.num_elem, st.num_node
+
+    def set_fused_multiply_add(self, enable: bool) -> None:
+        self._check(lib().lulesh_set_fused_multiply_add(self.h, int(bool(enable))))
+        self.fused = bool(enable)
 
     @classmethod
     def from_lmesh(cls, path: str, stoptime: float = 1.0e-2,
@@ -306,7 +318,7 @@ class Domain:
     def _wrap(cls, h, ic_mode: int) -> "Domain":
         d = cls.__new__(cls)
         d.h = h
-        d.nx, d.ic_mode, d.mixed = 0, ic_mode, False
+        d.nx, d.ic_mode, d.mixed, d.fused = 0, ic_mode, False, False
         st = d.stats()
         d.num_elem, d.num_node = st.num_elem, st.num_node
         d.origin = lib().lulesh_origin_element(h)
