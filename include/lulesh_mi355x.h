@@ -306,8 +306,10 @@ int lulesh_batch_get_shape(lulesh_batch *b, int *tx, int *ty, int *nz,
                            int *cycles_per_graph);
 
 /* The autotuner: times trial_cycles cycles of each candidate launch shape
- * from a saved copy of the state, restores the state and keeps the fastest
- * shape.  *best_ms (may be NULL) receives its milliseconds per cycle. */
+ * (plain launches after one warm-up cycle) from a saved copy of the state,
+ * restores the state and keeps the fastest shape with the batch's cycles
+ * per graph.  *best_ms (may be NULL) receives its milliseconds per cycle.
+ * The default shape is its pick on the 64-member ensemble of nx 16..64. */
 int lulesh_batch_autotune(lulesh_batch *b, int trial_cycles, double *best_ms);
 
 /* Milliseconds between the HIP events around the last step/run call. */

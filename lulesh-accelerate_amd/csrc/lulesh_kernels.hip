@@ -805,6 +805,49 @@ __device__ __forceinline__ void eos_extra_evaluations(const EosIn &a, int extra,
 
 }  // namespace
 
+namespace {
+
+// One block of 256 elements in mesh order, el the thread's element; every
+// thread of the block calls it (the minima are reduced over the block).
+__device__ __forceinline__ void q_eos_dt_nat_block(const DevFields &f, DevScalars *s,
+                                                   const PhysParams &prm, int el,
+                                                   unsigned long long *lds_c,
+                                                   unsigned long long *lds_h, RepLds &L)
+{
+    const bool valid = el < f.num_elem;
+    unsigned long long cand_c = as_bits(kBig), cand_h = as_bits(kBig);
+    const EosConst ec = eos_const(prm);
+    QInputs in;
+    EosIn a{1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    EosOut o;
+    int extra = 0;
+    if (valid) {
+        const int rep = f.rep[el];
+        extra = rep - 1;
+        int i, j, k;
+        elem_ijk(f, el, i, j, k);
+        q_inputs(f, prm, el, i, j, k, in);
+        double qlin, qquad;
+        monotonic_q(prm, in, qlin, qquad);
+        a = eos_prepare(f, s, prm, el, in, qlin, qquad);
+    }
+    __builtin_amdgcn_s_setprio(0);  // the evaluations yield to waves still loading
+    if (ec.unit_rho0) {
+        o = eos_repeat<true>(a, 1, ec);
+        eos_extra_evaluations<true>(a, extra, ec, L);
+    } else {
+        o = eos_repeat<false>(a, 1, ec);
+        eos_extra_evaluations<false>(a, extra, ec, L);
+    }
+    __builtin_amdgcn_s_setprio(1);  // stores and the block's reduction
+    if (valid) {
+        eos_finish(f, prm, el, in, a.vnewc, o, cand_c, cand_h);
+    }
+    fold_minima(s, cand_c, cand_h, lds_c, lds_h);
+}
+
+}  // namespace
+
 // Monotonic Q, EOS, volume update and time constraints with the elements in
 // mesh order, one per thread: the limiter's neighbour values and the
 // element's own fields are read in the order they lie in memory, and the
@@ -825,38 +868,8 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_nat(DevFields f, DevScalars *s
     }
     __builtin_amdgcn_s_setprio(1);  // the element's loads first
     const int b = xcd_contiguous_block(int(blockIdx.x), int(gridDim.x));
-    const int el = b * int(blockDim.x) + int(threadIdx.x);
-    const bool valid = el < f.num_elem;
-    unsigned long long cand_c = as_bits(kBig), cand_h = as_bits(kBig);
-    const EosConst ec = eos_const(prm);
-    QInputs in;
-    EosIn a{1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    EosOut o;
-    int extra = 0;
-    if (valid) {
-        const int rep = f.rep[el];
-        extra = rep - 1;
-        int i, j, k;
-        elem_ijk(f, el, i, j, k);
-        q_inputs(f, prm, el, i, j, k, in);
-        double qlin, qquad;
-        monotonic_q(prm, in, qlin, qquad);
-        a = eos_prepare(f, s, prm, el, in, qlin, qquad);
-    }
-    __builtin_amdgcn_s_setprio(0);  // the evaluations yield to waves still loading
-    RepLds &L = rep_lds[threadIdx.x >> 6];
-    if (ec.unit_rho0) {
-        o = eos_repeat<true>(a, 1, ec);
-        eos_extra_evaluations<true>(a, extra, ec, L);
-    } else {
-        o = eos_repeat<false>(a, 1, ec);
-        eos_extra_evaluations<false>(a, extra, ec, L);
-    }
-    __builtin_amdgcn_s_setprio(1);  // stores and the block's reduction
-    if (valid) {
-        eos_finish(f, prm, el, in, a.vnewc, o, cand_c, cand_h);
-    }
-    fold_minima(s, cand_c, cand_h, lds_c, lds_h);
+    q_eos_dt_nat_block(f, s, prm, b * int(blockDim.x) + int(threadIdx.x), lds_c, lds_h,
+                       rep_lds[threadIdx.x >> 6]);
 }
 
 // persistent blocks over the element chunks of all members; a chunk lies
@@ -867,6 +880,7 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_batch(const BatchMember *__res
 {
     __shared__ unsigned long long lds_c[4], lds_h[4];
     __shared__ int skip;
+    __shared__ RepLds rep_lds[4];
     const int n = work->num_chunks;
     for (int c = int(blockIdx.x); c < n; c += int(gridDim.x)) {
         const int2 w = work->chunks[c];
@@ -882,8 +896,10 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_batch(const BatchMember *__res
         if (skip) {
             continue;
         }
-        q_eos_dt_block(m.f, m.s, prm, __builtin_amdgcn_readfirstlane(w.y) + int(threadIdx.x),
-                       lds_c, lds_h);
+        __builtin_amdgcn_s_setprio(1);  // the element's loads first
+        q_eos_dt_nat_block(m.f, m.s, prm,
+                           __builtin_amdgcn_readfirstlane(w.y) + int(threadIdx.x), lds_c,
+                           lds_h, rep_lds[threadIdx.x >> 6]);
     }
 }
 
