@@ -153,11 +153,9 @@ def bench_batched(lulesh_amd, plain_value: float) -> dict:
     b = lulesh_amd.Batch(nx, ef)
     t_build = time.perf_counter() - t0
     print(f"bench: ensemble built in {t_build:.1f} s", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    tuned_ms = b.autotune(4)
-    t_tune = time.perf_counter() - t0
+    # the default launch shape (the autotuner's pick on this ensemble, set
+    # in the library), no tuning at run time
     shape = b.shape()
-    print(f"bench: autotuned in {t_tune:.1f} s: {shape}", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     rc = b.run()
     wall = time.perf_counter() - t0
@@ -165,6 +163,14 @@ def bench_batched(lulesh_amd, plain_value: float) -> dict:
     cycles = [b.stats(i)[1].cycle for i in range(b.n)]
     b.close()
     print(f"bench: ensemble ran in {wall:.2f} s", file=sys.stderr, flush=True)
+    # the autotuner on a fresh copy of the ensemble: its time and its pick
+    b = lulesh_amd.Batch(nx, ef)
+    t0 = time.perf_counter()
+    tuned_ms = b.autotune(4)
+    t_tune = time.perf_counter() - t0
+    tuned_shape = b.shape()
+    b.close()
+    print(f"bench: autotuned in {t_tune:.2f} s: {tuned_shape}", file=sys.stderr, flush=True)
     zc_plain, wall_plain = 0, 0.0
     for n, f in zip(nx, ef):
         d = lulesh_amd.Domain(int(n), efactor=float(f))
@@ -183,12 +189,13 @@ def bench_batched(lulesh_amd, plain_value: float) -> dict:
                     "11 regions, each to t=1e-2",
         "members": int(len(nx)), "zones_total": int(sum(int(n) ** 3 for n in nx)),
         "zone_cycles": zc, "member_cycles": [int(min(cycles)), int(max(cycles))],
-        "wall_s": wall, "build_s": t_build, "autotune_s": t_tune,
-        "autotune_ms_per_cycle": tuned_ms, "shape": shape,
+        "wall_s": wall, "build_s": t_build, "shape": shape,
+        "autotune": {"s": t_tune, "ms_per_cycle": tuned_ms, "shape": tuned_shape,
+                     "same_as_default": tuned_shape == shape},
         "same_sizes_single_domain_path": {"value": zc_plain / wall_plain,
                                           "wall_s": wall_plain},
         "vs_single_domain_path": value / (zc_plain / wall_plain),
-        # the same with the autotuner's time counted in the ensemble's
+        # as if the autotuner had run before the ensemble
         "vs_single_domain_path_with_autotune": wall_plain / (wall + t_tune),
         "vs_plain_150": value / plain_value,
     }
@@ -292,13 +299,15 @@ def bench_unstructured(lulesh_amd, nx: int = 150, steps: int = 50, warmup: int =
 
 def bench_mixed(lulesh_amd, nx: int = 120) -> dict:
     """The mixed-precision form on its workload: Sedov at nx^3 (LULESH 2.0
-    ICs, 11 regions), run to t = 1e-2 in float64 and in mixed precision;
-    throughput of both over the whole run and the mixed run's errors
-    against the float64 one (origin energy, total energy, shock radius)."""
+    ICs, 11 regions), run to t = 1e-2 in float64 (bit-identical), in mixed
+    precision and in float64 with fused multiply-adds; throughput of each
+    over the whole run and the mixed run's errors against the bit-identical
+    float64 one (origin energy, total energy, shock radius)."""
     out = {}
     doms = {}
-    for name, mixed in (("fp64", False), ("mixed", True)):
-        d = lulesh_amd.Domain(nx, mixed=mixed)
+    for name, mixed, fused in (("fp64", False, False), ("mixed", True, False),
+                               ("fp64_fused", False, True)):
+        d = lulesh_amd.Domain(nx, mixed=mixed, fused=fused)
         d.step(1)  # first launch of each kernel outside the timed run
         t0 = time.perf_counter()
         rc, n = d.run()
@@ -308,6 +317,7 @@ def bench_mixed(lulesh_amd, nx: int = 120) -> dict:
         doms[name] = d
         print(f"bench: {name} {nx}^3 to t=1e-2: {st.cycle} cycles in {wall:.2f} s",
               file=sys.stderr, flush=True)
+    doms.pop("fp64_fused").close()
     r, m = doms["fp64"], doms["mixed"]
     e_r, e_m = r.stats().e_origin, m.stats().e_origin
     t_r, t_m = r.total_energy(), m.total_energy()
@@ -317,8 +327,11 @@ def bench_mixed(lulesh_amd, nx: int = 120) -> dict:
     return {
         "value": out["mixed"]["value"], "unit": "zones·cycles/s",
         "workload": f"sedov_{nx}^3_lulesh2.0_regions11 to t=1e-2",
-        "fp64": out["fp64"], "mixed": out["mixed"],
+        "fp64": out["fp64"], "mixed": out["mixed"], "fp64_fused": out["fp64_fused"],
+        # against the bit-identical float64 step (the errors' reference) and
+        # the float64 step with fused multiply-adds
         "vs_fp64_same_workload": out["mixed"]["value"] / out["fp64"]["value"],
+        "vs_fp64_fused_same_workload": out["mixed"]["value"] / out["fp64_fused"]["value"],
         "errors_vs_fp64": {
             "origin_energy_rel": abs(e_m - e_r) / abs(e_r),
             "total_energy_rel": abs(t_m - t_r) / abs(t_r),
