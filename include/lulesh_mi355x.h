@@ -194,10 +194,14 @@ int lulesh_set_parameter(lulesh_domain *d, const char *name, double value);
  * leaves room for.  Only the shapes the library was built with are accepted
  * (LULESH_BAD_ARG otherwise); results do not depend on the shape. */
 int lulesh_set_sweep_shape(lulesh_domain *d, int tx, int ty, int nz, int waves);
-/* Schedule of the Q/EOS kernel (float64 and mixed domains): consecutive
- * blocks take element chunks from `bands` bands of the region-sorted EOS
- * order in turn (1 = costly regions first, then the rest).  A tuning knob
- * with no counterpart in the reference; results do not depend on it. */
+/* Schedule of the Q/EOS kernel (float64 and mixed domains).  0, the
+ * default: blocks of consecutive elements in mesh order, each block sharing
+ * its elements' EOS repetitions among its threads, the costliest blocks
+ * first.  bands > 0: consecutive blocks take element
+ * chunks from `bands` bands of the region-sorted EOS order in turn (1 =
+ * costly regions first, then the rest; odd counts spread the costly band
+ * over the 8 XCDs).  A tuning knob with no counterpart in the reference;
+ * results do not depend on it. */
 int lulesh_set_eos_bands(lulesh_domain *d, int bands);
 int lulesh_get_sweep_shape(lulesh_domain *d, int *tx, int *ty, int *nz,
                            int *waves);

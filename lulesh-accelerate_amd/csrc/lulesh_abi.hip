@@ -49,14 +49,8 @@ constexpr SweepShape kDefaultSweep = {32, 8, 17, 2};
 // on a CU
 constexpr SweepShape kDefaultSweepMixed = {32, 8, 12, 3};
 
-// consecutive Q/EOS blocks take chunks from this many bands of the
-// region-sorted schedule (odd: blocks go to the 8 XCDs round-robin, and an
-// even count would send the costly band to only some of them; 3 and 5 are
-// 8% faster than 1 at 150^3, 2, 4 and 8 up to 2.5x slower);
-// lulesh_set_eos_bands overrides it
-constexpr int kDefaultEosBands = 3;
-
 constexpr int kBlock = 256;
+static_assert(kBlock == kEosBlockElems, "the Q/EOS block order is cut for this block size");
 constexpr int kBatch = 32;  // cycles queued per host round trip in run()
 
 // Node planes per block of the default sweep on an nx^3 domain: the largest
@@ -108,7 +102,7 @@ struct lulesh_domain {
     DevMeshU mesh{};
     int origin = 0;             // the element holding the initial energy
     std::vector<void *> bufs;   // device allocations of an unstructured domain
-    int eos_bands = kDefaultEosBands;   // EOS schedule interleave (eos_chunk)
+    int eos_bands = 0;   // 0: mesh-order Q/EOS; else the interleave of eos_chunk
     const MlpFrags *learned = nullptr;  // learned-Q network, if set
     float learned_z_max = 0.f;
     int learned_form = 0;               // LULESH_LEARNED_FACTORS / _STANDIN
@@ -271,7 +265,10 @@ int queue_cycle(lulesh_domain *d)
          })))
         return rc;
     if ((rc = launch(d, KID_Q_EOS_DT, [&] {
-             if (mixed) {
+             if (mixed && d->eos_bands == 0) {
+                 hipLaunchKernelGGL(k_q_eos_dt_mixed_nat, dim3(ge), dim3(kBlock), 0, st, fm,
+                                    s, p);
+             } else if (mixed) {
                  hipLaunchKernelGGL(k_q_eos_dt_mixed, dim3(ge), dim3(kBlock), 0, st, fm, s,
                                     p, d->eos_bands);
              } else if (d->learned) {
@@ -437,7 +434,6 @@ int create_domain(const lulesh_config *cfg, bool mixed, double efactor,
     d->cfg = *cfg;
     d->mixed = mixed;
     d->sweep = mixed ? kDefaultSweepMixed : kDefaultSweep;
-    d->eos_bands = mixed ? kDefaultEosBands : 0;
     if (!mixed && cfg->nx >= 1 && cfg->nx <= 600) {
         d->sweep.nz = default_sweep_planes(cfg->nx);
     }
@@ -962,9 +958,6 @@ int lulesh_set_eos_bands(lulesh_domain *d, int bands)
 {
     if (!d || bands < 0 || bands > 4096) {
         return LULESH_BAD_ARG;
-    }
-    if (bands == 0 && d->mixed) {
-        return fail(d, LULESH_BAD_ARG, "the mixed-precision Q/EOS has no mesh-order schedule");
     }
     d->eos_bands = bands;
     return LULESH_OK;

@@ -78,18 +78,6 @@ __device__ __forceinline__ void apply_constraints(DevScalars *s, int ic_mode)
     if (h < kBig) s->dthydro = h;
 }
 
-// Block b of g in the mesh order the blocks of a natural-order kernel walk:
-// the hardware deals the blocks of a launch out to the 8 XCDs in turn
-// (block b to XCD b % 8), so XCD r takes the r-th of 8 contiguous ranges of
-// blocks, in order, and the planes one element layer shares with the next
-// stay in that XCD's L2.  A bijection of [0, g).
-__device__ __forceinline__ int xcd_contiguous_block(int b, int g)
-{
-    const int q = g >> 3, rem = g & 7;
-    const int r = b & 7, i = b >> 3;
-    return r * q + min(r, rem) + i;
-}
-
 // element (i, j, k) of a flat element index
 __device__ __forceinline__ void elem_ijk(const DevFields &f, int el, int &i, int &j,
                                          int &k)
@@ -741,56 +729,63 @@ __global__ __launch_bounds__(256) void k_q_eos_dt(DevFields f, DevScalars *s,
 
 namespace {
 
-// Per wave: the EOS inputs of each lane's element and the inclusive prefix of
-// the evaluations beyond each element's first.
+// Per block: the EOS inputs of each thread's element, the inclusive prefix of
+// the evaluations beyond each element's first, and each wave's total.
 struct RepLds {
-    double in[7][64];
-    int end[64];
+    double in[7][256];
+    int end[256];
+    int wave_total[4];
 };
 
-// The EOS evaluations of a wave's 64 elements beyond each one's first.  The
-// lane that owns an element evaluates it once (its result is the element's);
-// the other rep - 1 evaluations of every element are dealt out over all 64
-// lanes, so a wave of elements with 1, 2 and 20 repetitions takes
-// ceil(sum(rep - 1) / 64) extra passes instead of running every lane 20
-// times.  An element's evaluations all read the same inputs and give the
-// same results: each is computed in full (its inputs pass through an empty
-// asm statement) and its result marked as used, as the reference repeats the
-// evaluation (lulesh-2.0.3/lulesh.cc:2413-2424).  Every lane of the wave
-// calls it.
+// The EOS evaluations of a block's 256 elements beyond each one's first.  The
+// thread that owns an element evaluates it once (its result is the
+// element's); the other rep - 1 evaluations of every element are dealt out
+// over all 256 threads, so a block of elements with 1, 2 and 20 repetitions
+// takes ceil(sum(rep - 1) / 256) extra passes instead of running every
+// thread 20 times.  An element's evaluations all read the same inputs and
+// give the same results: each is computed in full (its inputs pass through
+// an empty asm statement) and its result marked as used, as the reference
+// repeats the evaluation (lulesh-2.0.3/lulesh.cc:2413-2424).  Every thread
+// of the block (256, four waves) calls it.
 template <bool Unit>
 __device__ __forceinline__ void eos_extra_evaluations(const EosIn &a, int extra,
                                                       const EosConst &ec, RepLds &L)
 {
-    const int lane = int(threadIdx.x) & 63;
+    const int t = int(threadIdx.x), lane = t & 63, wave = t >> 6;
     int incl = extra;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
         const int o = __shfl_up(incl, off, 64);
         if (lane >= off) incl += o;
     }
-    const int total = __shfl(incl, 63, 64);
-    if (total == 0) {
+    if (lane == 63) L.wave_total[wave] = incl;
+    __syncthreads();
+    int base = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        const int v = L.wave_total[w];
+        if (w < wave) base += v;
+        total += v;
+    }
+    if (total == 0) {  // the same for every thread of the block
         return;
     }
-    L.end[lane] = incl;
-    L.in[0][lane] = a.vnewc;
-    L.in[1][lane] = a.e_old;
-    L.in[2][lane] = a.delvc;
-    L.in[3][lane] = a.p_old;
-    L.in[4][lane] = a.q_old;
-    L.in[5][lane] = a.qquad;
-    L.in[6][lane] = a.qlin;
-    // the wave reads what its lanes wrote: LDS operations of one wave are
-    // performed in order, and this keeps the compiler from moving them
-    __builtin_amdgcn_wave_barrier();
-    for (int w = lane; w < total; w += 64) {
-        // the lane whose evaluations [end - extra, end) hold number w: the
-        // count of lanes whose range ends at or before w (end is
-        // non-decreasing and end[63] = total > w, so src <= 63)
+    L.end[t] = incl + base;
+    L.in[0][t] = a.vnewc;
+    L.in[1][t] = a.e_old;
+    L.in[2][t] = a.delvc;
+    L.in[3][t] = a.p_old;
+    L.in[4][t] = a.q_old;
+    L.in[5][t] = a.qquad;
+    L.in[6][t] = a.qlin;
+    __syncthreads();
+    for (int w = t; w < total; w += 256) {
+        // the thread whose evaluations [end - extra, end) hold number w: the
+        // count of threads whose range ends at or before w (end is
+        // non-decreasing and end[255] = total > w, so src <= 255)
         int src = 0;
 #pragma unroll
-        for (int step = 32; step >= 1; step >>= 1) {
+        for (int step = 128; step >= 1; step >>= 1) {
             if (L.end[src + step - 1] <= w) src += step;
         }
         double b0 = L.in[0][src], b1 = L.in[1][src], b2 = L.in[2][src], b3 = L.in[3][src];
@@ -800,7 +795,6 @@ __device__ __forceinline__ void eos_extra_evaluations(const EosIn &a, int extra,
         const EosOut o = eos_element<Unit>(b0, b1, b2, b3, b4, b5, b6, ec);
         asm volatile("" ::"v"(o.p), "v"(o.e), "v"(o.q), "v"(o.ss));
     }
-    __builtin_amdgcn_wave_barrier();
 }
 
 }  // namespace
@@ -849,16 +843,18 @@ __device__ __forceinline__ void q_eos_dt_nat_block(const DevFields &f, DevScalar
 }  // namespace
 
 // Monotonic Q, EOS, volume update and time constraints with the elements in
-// mesh order, one per thread: the limiter's neighbour values and the
-// element's own fields are read in the order they lie in memory, and the
-// blocks of each XCD walk one contiguous part of the mesh.  The regions'
-// EOS repetitions are balanced within each wave (eos_extra_evaluations).
+// mesh order, one per thread: each block takes 256 consecutive elements, so
+// the limiter's neighbour values and the element's own fields are read in
+// the order they lie in memory.  The regions' EOS repetitions are shared
+// over each block's threads (eos_extra_evaluations), and the blocks run in
+// the order f.eos_blocks gives (costliest first, then the rest in mesh
+// order, each XCD on one contiguous part of the mesh; eos_block_order).
 __global__ __launch_bounds__(256) void k_q_eos_dt_nat(DevFields f, DevScalars *s,
                                                       PhysParams prm)
 {
     __shared__ unsigned long long lds_c[4], lds_h[4];
     __shared__ int skip;
-    __shared__ RepLds rep_lds[4];
+    __shared__ RepLds rep_lds;
     if (threadIdx.x == 0) {
         skip = (s->active == 0 || s->error != 0) ? 1 : 0;
     }
@@ -867,9 +863,9 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_nat(DevFields f, DevScalars *s
         return;
     }
     __builtin_amdgcn_s_setprio(1);  // the element's loads first
-    const int b = xcd_contiguous_block(int(blockIdx.x), int(gridDim.x));
+    const int b = f.eos_blocks[blockIdx.x];
     q_eos_dt_nat_block(f, s, prm, b * int(blockDim.x) + int(threadIdx.x), lds_c, lds_h,
-                       rep_lds[threadIdx.x >> 6]);
+                       rep_lds);
 }
 
 // persistent blocks over the element chunks of all members; a chunk lies
@@ -880,7 +876,7 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_batch(const BatchMember *__res
 {
     __shared__ unsigned long long lds_c[4], lds_h[4];
     __shared__ int skip;
-    __shared__ RepLds rep_lds[4];
+    __shared__ RepLds rep_lds;
     const int n = work->num_chunks;
     for (int c = int(blockIdx.x); c < n; c += int(gridDim.x)) {
         const int2 w = work->chunks[c];
@@ -899,7 +895,7 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_batch(const BatchMember *__res
         __builtin_amdgcn_s_setprio(1);  // the element's loads first
         q_eos_dt_nat_block(m.f, m.s, prm,
                            __builtin_amdgcn_readfirstlane(w.y) + int(threadIdx.x), lds_c,
-                           lds_h, rep_lds[threadIdx.x >> 6]);
+                           lds_h, rep_lds);
     }
 }
 

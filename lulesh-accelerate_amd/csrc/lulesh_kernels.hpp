@@ -127,6 +127,8 @@ struct DevFields {
     const uint16_t *eos_rep;
     // EOS repetitions of each element, in mesh order
     const uint16_t *rep;
+    // dispatch order of the mesh-order Q/EOS blocks (eos_block_order)
+    const int32_t *eos_blocks;
 };
 
 // The mixed-precision domain: the node arrays, the energy, the volumes and
@@ -145,6 +147,7 @@ struct DevFieldsMixed {
     const int32_t *eos_order;
     const uint16_t *eos_rep;
     const uint16_t *rep;
+    const int32_t *eos_blocks;
 };
 
 // Launch shape of the force sweep: TX x TY element columns per block (one
@@ -258,6 +261,7 @@ __global__ void k_q_eos_dt_learned(DevFields f, DevScalars *s, PhysParams prm,
 // The mixed-precision phases (the time step kernels are shared).
 __global__ void k_kinematics_q_mixed(DevFieldsMixed f, DevScalars *s, PhysParams prm);
 __global__ void k_q_eos_dt_mixed(DevFieldsMixed f, DevScalars *s, PhysParams prm, int bands);
+__global__ void k_q_eos_dt_mixed_nat(DevFieldsMixed f, DevScalars *s, PhysParams prm);
 hipError_t launch_force_sweep_mixed(const SweepShape &shape, const DevFieldsMixed &f,
                                     DevScalars *s, const PhysParams &prm,
                                     hipStream_t stream);

@@ -39,7 +39,16 @@ struct HostMesh {
     std::vector<uint16_t> eos_rep;      // EOS repetitions at each eos_order slot
     std::vector<int32_t> region_offset; // start of each region in eos_order
     std::vector<uint16_t> elem_rep;     // EOS repetitions of each element
+    std::vector<int32_t> eos_blocks;    // dispatch order of the mesh-order Q/EOS blocks
 };
+
+// Elements per block of the mesh-order Q/EOS kernels (one per thread).
+constexpr int kEosBlockElems = 256;
+
+// The order in which the blocks of a mesh-order Q/EOS launch take the mesh's
+// blocks of kEosBlockElems consecutive elements (eos_block_order in
+// lulesh_mesh.cpp): slot t of the launch processes block order[t].
+std::vector<int32_t> eos_block_order(const std::vector<uint16_t> &rep);
 
 // Builds the mesh, masses, ICs and regions; throws std::invalid_argument.
 HostMesh build_mesh(const MeshConfig &cfg);

@@ -715,8 +715,258 @@ __global__ __launch_bounds__(256) void k_kinematics_q_mixed(DevFieldsMixed f,
 
 // ---------------------------------------------------------------------------
 // Monotonic Q, EOS, volume update and time constraints; the structure of
-// k_q_eos_dt (lulesh_kernels.hip)
+// k_q_eos_dt and k_q_eos_dt_nat (lulesh_kernels.hip)
 // ---------------------------------------------------------------------------
+namespace {
+
+struct EosInMixed {
+    double vnewc, e_old;
+    float delvc, p_old, q_old, qquad, qlin;
+};
+
+// what an element keeps for its stores after the EOS
+struct QKeepMixed {
+    double vnew;
+    float vdov, arealg;
+};
+
+__device__ __forceinline__ EosConstMixed eos_const_mixed(const PhysParams &prm)
+{
+    EosConstMixed ec;
+    ec.e_cut = prm.e_cut;
+    ec.emin = prm.emin;
+    ec.eosvmax = prm.eosvmax;
+    ec.eosvmin = prm.eosvmin;
+    ec.p_cut = float(prm.p_cut);
+    ec.q_cut = float(prm.q_cut);
+    ec.pmin = float(prm.pmin);
+    ec.rho0_inv = float(1.0 / prm.refdens);
+    ec.unit_rho0 = prm.refdens == 1.0;
+    ec.ss_tiny = float(prm.ss_tiny);
+    ec.ss_floor = float(prm.ss_floor);
+    return ec;
+}
+
+// The element's monotonic Q (ql / qq stored) and its side of the EOS before
+// the evaluations: qstop tested on the previous cycle's q, the new and old
+// relative volumes clamped and the clamped old volume tested
+// (reference/C/LULESH-OMP.cc:2183-2320, :2368-2381; lulesh-2.0.3/
+// lulesh.cc:2704-2753).  The element's inputs are all loaded before its
+// first store.
+__device__ __forceinline__ EosInMixed q_eos_prepare_mixed(const DevFieldsMixed &f,
+                                                          DevScalars *s, const PhysParams &prm,
+                                                          int el, QKeepMixed &kp)
+{
+    const int nx = f.nx;
+    int i, j, k;
+    elem_ijk(f, el, i, j, k);
+    const int sy = nx, sz = nx * nx;
+
+    const float dvxi = f.delv_xi[el], dveta = f.delv_eta[el], dvzeta = f.delv_zeta[el];
+    // neighbours across a boundary face come from the element itself and
+    // are then replaced
+    const float xm = f.delv_xi[(i == 0) ? el : el - 1];
+    const float xp = f.delv_xi[(i == nx - 1) ? el : el + 1];
+    const float ym = f.delv_eta[(j == 0) ? el : el - sy];
+    const float yp = f.delv_eta[(j == nx - 1) ? el : el + sy];
+    const float zm = f.delv_zeta[(k == 0) ? el : el - sz];
+    const float zp = f.delv_zeta[(k == nx - 1) ? el : el + sz];
+    const float vdov = f.vdov[el];
+    const double vnew = f.vnew[el];
+    const double volo = f.volo[el];
+    const float q_old = f.q[el];
+    const double v_old = f.v[el];
+    const double e_old = f.e[el];
+    const float delvc = f.delv[el], p_old = f.p[el];
+    const float arealg = (vdov != 0.f) ? f.arealg[el] : 0.f;
+    const float lm = float(prm.monoq_limiter_mult), ms = float(prm.monoq_max_slope);
+    const float phixi = mono_phi(dvxi, (i == 0) ? dvxi : xm, (i == nx - 1) ? 0.f : xp, lm, ms);
+    const float phieta = mono_phi(dveta, (j == 0) ? dveta : ym, (j == nx - 1) ? 0.f : yp, lm, ms);
+    const float phizeta =
+        mono_phi(dvzeta, (k == 0) ? dvzeta : zm, (k == nx - 1) ? 0.f : zp, lm, ms);
+    float qlin, qquad;
+    if (vdov > 0.f) {
+        qlin = 0.f;
+        qquad = 0.f;
+    } else {
+        float dxi = dvxi * f.delx_xi[el];
+        float deta = dveta * f.delx_eta[el];
+        float dzeta = dvzeta * f.delx_zeta[el];
+        if (dxi > 0.f) dxi = 0.f;
+        if (deta > 0.f) deta = 0.f;
+        if (dzeta > 0.f) dzeta = 0.f;
+        const float rho = float(f.elem_mass[el]) * frcp(float(volo * vnew));
+        qlin = -float(prm.qlc_monoq) * rho *
+               (dxi * (1.f - phixi) + deta * (1.f - phieta) + dzeta * (1.f - phizeta));
+        qquad = float(prm.qqc_monoq) * rho *
+                (dxi * dxi * (1.f - phixi * phixi) + deta * deta * (1.f - phieta * phieta) +
+                 dzeta * dzeta * (1.f - phizeta * phizeta));
+    }
+    f.ql[el] = qlin;
+    f.qq[el] = qquad;
+
+    if (double(q_old) > prm.qstop) {
+        raise_error(s, -2);
+    }
+    double vnewc = vnew;
+    if (prm.eosvmin != 0.) {
+        if (vnewc < prm.eosvmin) vnewc = prm.eosvmin;
+    }
+    if (prm.eosvmax != 0.) {
+        if (vnewc > prm.eosvmax) vnewc = prm.eosvmax;
+    }
+    double vc = v_old;
+    if (prm.eosvmin != 0.) {
+        if (vc < prm.eosvmin) vc = prm.eosvmin;
+    }
+    if (prm.eosvmax != 0.) {
+        if (vc > prm.eosvmax) vc = prm.eosvmax;
+    }
+    if (vc <= 0.) {
+        raise_error(s, -1);
+    }
+    kp.vnew = vnew;
+    kp.vdov = vdov;
+    kp.arealg = arealg;
+    return EosInMixed{vnewc, e_old, delvc, p_old, q_old, qquad, qlin};
+}
+
+// `rep` evaluations of the EOS on the same inputs, as the reference repeats
+// them (lulesh-2.0.3/lulesh.cc:2413-2424): the inputs pass through an empty
+// asm statement at the top of every repetition, carried in the same
+// registers, so each repetition is computed in full, and every result is
+// marked as used.
+__device__ __forceinline__ EosOutMixed eos_repeat_mixed(const EosInMixed &a, int rep,
+                                                        const EosConstMixed &ec)
+{
+    double a0 = a.vnewc, a1 = a.e_old;
+    float a2 = a.delvc, a3 = a.p_old, a4 = a.q_old, a5 = a.qquad, a6 = a.qlin;
+    EosOutMixed o;
+    o.e = 0.0;
+    o.p = o.q = o.ss = 0.f;
+    for (int r = 0; r < rep; ++r) {
+        asm volatile("" : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(a4), "+v"(a5),
+                     "+v"(a6));
+        o = eos_element_mixed(a0, a1, a2, a3, a4, a5, a6, ec);
+        asm volatile("" ::"v"(o.p), "v"(o.e), "v"(o.q), "v"(o.ss));
+    }
+    return o;
+}
+
+// The element's side after the EOS: p, e, q, ss stored, the volume update
+// (1.0 keeps the unclamped vnew, 2.0 the clamped one, lulesh-2.0.3/
+// lulesh.cc:2439-2452) and the Courant and hydro candidates (:2799-2933).
+__device__ __forceinline__ void eos_finish_mixed(const DevFieldsMixed &f,
+                                                 const PhysParams &prm, int el,
+                                                 const QKeepMixed &kp, double vnewc,
+                                                 const EosOutMixed &o,
+                                                 unsigned long long &cand_c,
+                                                 unsigned long long &cand_h)
+{
+    f.p[el] = o.p;
+    f.e[el] = o.e;
+    f.q[el] = o.q;
+    f.ss[el] = o.ss;
+    double tmpv = (prm.ic_mode == 1) ? kp.vnew : vnewc;
+    if (fabs(tmpv - 1.0) < prm.v_cut) tmpv = 1.0;
+    f.v[el] = tmpv;
+    if (prm.ic_mode == 2) {
+        f.vnew[el] = vnewc;
+    }
+    const float vdov = kp.vdov;
+    if (vdov != 0.f) {
+        const float qqc2 = float(64.0 * prm.qqc * prm.qqc);
+        float dtf = o.ss * o.ss;
+        if (vdov < 0.f) {
+            dtf = dtf + qqc2 * kp.arealg * kp.arealg * vdov * vdov;
+        }
+        cand_c = as_bits(double(kp.arealg * frsq(dtf)));
+        cand_h = as_bits(prm.dvovmax / (double(fabsf(vdov)) + 1.e-20));
+    }
+}
+
+__device__ __forceinline__ void fold_minima_mixed(DevScalars *s, unsigned long long cand_c,
+                                                  unsigned long long cand_h,
+                                                  unsigned long long *lds_c,
+                                                  unsigned long long *lds_h)
+{
+    const unsigned long long mc = block_min(cand_c, lds_c);
+    const unsigned long long mh = block_min(cand_h, lds_h);
+    if (threadIdx.x == 0) {
+        if (mc < as_bits(kBig)) atomicMin(&s->acc_courant, mc);
+        if (mh < as_bits(kBig)) atomicMin(&s->acc_hydro, mh);
+    }
+}
+
+// Per block: the EOS inputs of each thread's element, the inclusive prefix of
+// the evaluations beyond each element's first, and each wave's total.
+struct RepLdsMixed {
+    double in_d[2][256];
+    float in_f[5][256];
+    int end[256];
+    int wave_total[4];
+};
+
+// The EOS evaluations of a block's 256 elements beyond each one's first,
+// dealt out over all 256 threads (the scheme of eos_extra_evaluations in
+// lulesh_kernels.hip): the thread that owns an element evaluates it once and
+// keeps that result; the other rep - 1 evaluations of every element take
+// ceil(sum(rep - 1) / 256) passes of the block.  Each is computed in full
+// and its result marked as used.  Every thread of the block calls it.
+__device__ __forceinline__ void eos_extra_evaluations_mixed(const EosInMixed &a, int extra,
+                                                            const EosConstMixed &ec,
+                                                            RepLdsMixed &L)
+{
+    const int t = int(threadIdx.x), lane = t & 63, wave = t >> 6;
+    int incl = extra;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += o;
+    }
+    if (lane == 63) L.wave_total[wave] = incl;
+    __syncthreads();
+    int base = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        const int v = L.wave_total[w];
+        if (w < wave) base += v;
+        total += v;
+    }
+    if (total == 0) {  // the same for every thread of the block
+        return;
+    }
+    L.end[t] = incl + base;
+    L.in_d[0][t] = a.vnewc;
+    L.in_d[1][t] = a.e_old;
+    L.in_f[0][t] = a.delvc;
+    L.in_f[1][t] = a.p_old;
+    L.in_f[2][t] = a.q_old;
+    L.in_f[3][t] = a.qquad;
+    L.in_f[4][t] = a.qlin;
+    __syncthreads();
+    for (int w = t; w < total; w += 256) {
+        // the thread whose evaluations [end - extra, end) hold number w
+        // (end is non-decreasing and end[255] = total > w, so src <= 255)
+        int src = 0;
+#pragma unroll
+        for (int step = 128; step >= 1; step >>= 1) {
+            if (L.end[src + step - 1] <= w) src += step;
+        }
+        double b0 = L.in_d[0][src], b1 = L.in_d[1][src];
+        float b2 = L.in_f[0][src], b3 = L.in_f[1][src], b4 = L.in_f[2][src];
+        float b5 = L.in_f[3][src], b6 = L.in_f[4][src];
+        asm volatile("" : "+v"(b0), "+v"(b1), "+v"(b2), "+v"(b3), "+v"(b4), "+v"(b5),
+                     "+v"(b6));
+        const EosOutMixed o = eos_element_mixed(b0, b1, b2, b3, b4, b5, b6, ec);
+        asm volatile("" ::"v"(o.p), "v"(o.e), "v"(o.q), "v"(o.ss));
+    }
+}
+
+}  // namespace
+
+// The region-ordered schedule: block b takes chunk eos_chunk(b) of the
+// elements sorted by region, each thread running its element's repetitions.
 __global__ __launch_bounds__(256) void k_q_eos_dt_mixed(DevFieldsMixed f, DevScalars *s,
                                                         PhysParams prm, int bands)
 {
@@ -731,140 +981,59 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_mixed(DevFieldsMixed f, DevSca
     }
     const int slot = eos_chunk(int(blockIdx.x), int(gridDim.x), bands) * int(blockDim.x) +
                      int(threadIdx.x);
-    __builtin_amdgcn_s_setprio(1);
+    __builtin_amdgcn_s_setprio(1);  // the element's loads first
     unsigned long long cand_c = as_bits(kBig), cand_h = as_bits(kBig);
     if (slot < f.num_elem) {
         const int el = f.eos_order[slot];
         const int rep = f.eos_rep[slot];
-        const int nx = f.nx;
-        int i, j, k;
-        elem_ijk(f, el, i, j, k);
-        const int sy = nx, sz = nx * nx;
-
-        const float dvxi = f.delv_xi[el], dveta = f.delv_eta[el], dvzeta = f.delv_zeta[el];
-        // the element's inputs are loaded before its first store (as in
-        // k_q_eos_dt); neighbours across a boundary face come from the
-        // element itself and are then replaced
-        const float xm = f.delv_xi[(i == 0) ? el : el - 1];
-        const float xp = f.delv_xi[(i == nx - 1) ? el : el + 1];
-        const float ym = f.delv_eta[(j == 0) ? el : el - sy];
-        const float yp = f.delv_eta[(j == nx - 1) ? el : el + sy];
-        const float zm = f.delv_zeta[(k == 0) ? el : el - sz];
-        const float zp = f.delv_zeta[(k == nx - 1) ? el : el + sz];
-        const float vdov = f.vdov[el];
-        const double vnew = f.vnew[el];
-        const double volo = f.volo[el];
-        const float q_old = f.q[el];
-        const double v_old = f.v[el];
-        const double e_old = f.e[el];
-        const float delvc = f.delv[el], p_old = f.p[el];
-        const float arealg = (vdov != 0.f) ? f.arealg[el] : 0.f;
-        const float lm = float(prm.monoq_limiter_mult), ms = float(prm.monoq_max_slope);
-        const float phixi = mono_phi(dvxi, (i == 0) ? dvxi : xm, (i == nx - 1) ? 0.f : xp, lm, ms);
-        const float phieta =
-            mono_phi(dveta, (j == 0) ? dveta : ym, (j == nx - 1) ? 0.f : yp, lm, ms);
-        const float phizeta =
-            mono_phi(dvzeta, (k == 0) ? dvzeta : zm, (k == nx - 1) ? 0.f : zp, lm, ms);
-        float qlin, qquad;
-        if (vdov > 0.f) {
-            qlin = 0.f;
-            qquad = 0.f;
-        } else {
-            float dxi = dvxi * f.delx_xi[el];
-            float deta = dveta * f.delx_eta[el];
-            float dzeta = dvzeta * f.delx_zeta[el];
-            if (dxi > 0.f) dxi = 0.f;
-            if (deta > 0.f) deta = 0.f;
-            if (dzeta > 0.f) dzeta = 0.f;
-            const float rho = float(f.elem_mass[el]) * frcp(float(volo * vnew));
-            qlin = -float(prm.qlc_monoq) * rho *
-                   (dxi * (1.f - phixi) + deta * (1.f - phieta) + dzeta * (1.f - phizeta));
-            qquad = float(prm.qqc_monoq) * rho *
-                    (dxi * dxi * (1.f - phixi * phixi) + deta * deta * (1.f - phieta * phieta) +
-                     dzeta * dzeta * (1.f - phizeta * phizeta));
-        }
-        f.ql[el] = qlin;
-        f.qq[el] = qquad;
-
-        if (double(q_old) > prm.qstop) {
-            raise_error(s, -2);
-        }
-        double vnewc = vnew;
-        if (prm.eosvmin != 0.) {
-            if (vnewc < prm.eosvmin) vnewc = prm.eosvmin;
-        }
-        if (prm.eosvmax != 0.) {
-            if (vnewc > prm.eosvmax) vnewc = prm.eosvmax;
-        }
-        double vc = v_old;
-        if (prm.eosvmin != 0.) {
-            if (vc < prm.eosvmin) vc = prm.eosvmin;
-        }
-        if (prm.eosvmax != 0.) {
-            if (vc > prm.eosvmax) vc = prm.eosvmax;
-        }
-        if (vc <= 0.) {
-            raise_error(s, -1);
-        }
-
-        EosConstMixed ec;
-        ec.e_cut = prm.e_cut;
-        ec.emin = prm.emin;
-        ec.eosvmax = prm.eosvmax;
-        ec.eosvmin = prm.eosvmin;
-        ec.p_cut = float(prm.p_cut);
-        ec.q_cut = float(prm.q_cut);
-        ec.pmin = float(prm.pmin);
-        ec.rho0_inv = float(1.0 / prm.refdens);
-        ec.unit_rho0 = prm.refdens == 1.0;
-        ec.ss_tiny = float(prm.ss_tiny);
-        ec.ss_floor = float(prm.ss_floor);
-        EosOutMixed o;
-        o.e = 0.0;
-        o.p = o.q = o.ss = 0.f;
-        // every repetition of the region is evaluated: the inputs pass through
-        // an empty asm statement at the top of each one, carried in the same
-        // registers (see k_q_eos_dt)
-        // loads at wave priority 1, the repetitions at 0, the stores and the
-        // reduction at 1 (as in k_q_eos_dt)
-        __builtin_amdgcn_s_setprio(0);
-        double a0 = vnewc, a1 = e_old;
-        float a2 = delvc, a3 = p_old, a4 = q_old, a5 = qquad, a6 = qlin;
-        for (int r = 0; r < rep; ++r) {
-            asm volatile("" : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(a4),
-                         "+v"(a5), "+v"(a6));
-            o = eos_element_mixed(a0, a1, a2, a3, a4, a5, a6, ec);
-            asm volatile("" ::"v"(o.p), "v"(o.e), "v"(o.q), "v"(o.ss));
-        }
-        __builtin_amdgcn_s_setprio(1);
-        f.p[el] = o.p;
-        f.e[el] = o.e;
-        f.q[el] = o.q;
-        f.ss[el] = o.ss;
-
-        double tmpv = (prm.ic_mode == 1) ? vnew : vnewc;
-        if (fabs(tmpv - 1.0) < prm.v_cut) tmpv = 1.0;
-        f.v[el] = tmpv;
-        if (prm.ic_mode == 2) {
-            f.vnew[el] = vnewc;
-        }
-
-        if (vdov != 0.f) {
-            const float qqc2 = float(64.0 * prm.qqc * prm.qqc);
-            float dtf = o.ss * o.ss;
-            if (vdov < 0.f) {
-                dtf = dtf + qqc2 * arealg * arealg * vdov * vdov;
-            }
-            cand_c = as_bits(double(arealg * frsq(dtf)));
-            cand_h = as_bits(prm.dvovmax / (double(fabsf(vdov)) + 1.e-20));
-        }
+        QKeepMixed kp;
+        const EosInMixed a = q_eos_prepare_mixed(f, s, prm, el, kp);
+        const EosConstMixed ec = eos_const_mixed(prm);
+        __builtin_amdgcn_s_setprio(0);  // the repetitions yield to waves still loading
+        const EosOutMixed o = eos_repeat_mixed(a, rep, ec);
+        __builtin_amdgcn_s_setprio(1);  // stores and the block's reduction
+        eos_finish_mixed(f, prm, el, kp, a.vnewc, o, cand_c, cand_h);
     }
-    const unsigned long long mc = block_min(cand_c, lds_c);
-    const unsigned long long mh = block_min(cand_h, lds_h);
+    fold_minima_mixed(s, cand_c, cand_h, lds_c, lds_h);
+}
+
+// Mesh order, one element per thread, blocks of 256 consecutive elements in
+// the order f.eos_blocks gives, the regions' EOS repetitions shared over
+// each block's threads (as k_q_eos_dt_nat).  Each element's arithmetic is
+// that of k_q_eos_dt_mixed, so both give the same bits.
+__global__ __launch_bounds__(256) void k_q_eos_dt_mixed_nat(DevFieldsMixed f, DevScalars *s,
+                                                            PhysParams prm)
+{
+    __shared__ unsigned long long lds_c[4], lds_h[4];
+    __shared__ int skip;
+    __shared__ RepLdsMixed rep_lds;
     if (threadIdx.x == 0) {
-        if (mc < as_bits(kBig)) atomicMin(&s->acc_courant, mc);
-        if (mh < as_bits(kBig)) atomicMin(&s->acc_hydro, mh);
+        skip = (s->active == 0 || s->error != 0) ? 1 : 0;
     }
+    __syncthreads();
+    if (skip) {
+        return;
+    }
+    __builtin_amdgcn_s_setprio(1);  // the element's loads first
+    const int el = f.eos_blocks[blockIdx.x] * int(blockDim.x) + int(threadIdx.x);
+    const bool valid = el < f.num_elem;
+    const EosConstMixed ec = eos_const_mixed(prm);
+    unsigned long long cand_c = as_bits(kBig), cand_h = as_bits(kBig);
+    QKeepMixed kp{1.0, 0.f, 0.f};
+    EosInMixed a{1.0, 0.0, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int extra = 0;
+    if (valid) {
+        extra = int(f.rep[el]) - 1;
+        a = q_eos_prepare_mixed(f, s, prm, el, kp);
+    }
+    __builtin_amdgcn_s_setprio(0);  // the evaluations yield to waves still loading
+    const EosOutMixed o = eos_repeat_mixed(a, 1, ec);
+    eos_extra_evaluations_mixed(a, extra, ec, rep_lds);
+    __builtin_amdgcn_s_setprio(1);  // stores and the block's reduction
+    if (valid) {
+        eos_finish_mixed(f, prm, el, kp, a.vnewc, o, cand_c, cand_h);
+    }
+    fold_minima_mixed(s, cand_c, cand_h, lds_c, lds_h);
 }
 
 #define LULESH_MIXED_SHAPES(X) \

@@ -53,6 +53,7 @@ size_t domain_arena_bytes(const HostMesh &m)
     const size_t pn = round_up(size_t(m.num_node), 64);
     size_t bytes = (kNodeArrays * pn + kElemArrays * pe) * sizeof(double);
     bytes += round_up(pe * sizeof(int32_t), 256) + 2 * round_up(pe * sizeof(uint16_t), 256);
+    bytes += round_up(m.eos_blocks.size() * sizeof(int32_t), 256);
     bytes += round_up(sizeof(DevScalars), 256);
     return bytes;
 }
@@ -87,6 +88,8 @@ void carve_domain(char *base, const HostMesh &m, DevFields &f, DevScalars **s)
     sched += round_up(pe * sizeof(uint16_t), 256);
     f.rep = reinterpret_cast<uint16_t *>(sched);
     sched += round_up(pe * sizeof(uint16_t), 256);
+    f.eos_blocks = reinterpret_cast<int32_t *>(sched);
+    sched += round_up(m.eos_blocks.size() * sizeof(int32_t), 256);
     *s = reinterpret_cast<DevScalars *>(sched);
 }
 
@@ -107,6 +110,8 @@ hipError_t upload_mesh(const HostMesh &m, const DevFields &f, hipStream_t stream
         {const_cast<int32_t *>(f.eos_order), m.eos_order.data(), ne * sizeof(int32_t)},
         {const_cast<uint16_t *>(f.eos_rep), m.eos_rep.data(), ne * sizeof(uint16_t)},
         {const_cast<uint16_t *>(f.rep), m.elem_rep.data(), ne * sizeof(uint16_t)},
+        {const_cast<int32_t *>(f.eos_blocks), m.eos_blocks.data(),
+         m.eos_blocks.size() * sizeof(int32_t)},
     };
     for (const Copy &c : copies) {
         const hipError_t e =
@@ -148,6 +153,7 @@ size_t domain_arena_bytes_mixed(const HostMesh &m)
     size_t bytes = (kNodeArrays * pn + kElemArrays64 * pe) * sizeof(double) +
                    kElemArrays32 * pe * sizeof(float);
     bytes += round_up(pe * sizeof(int32_t), 256) + 2 * round_up(pe * sizeof(uint16_t), 256);
+    bytes += round_up(m.eos_blocks.size() * sizeof(int32_t), 256);
     bytes += round_up(sizeof(DevScalars), 256);
     return bytes;
 }
@@ -183,6 +189,8 @@ void carve_domain_mixed(char *base, const HostMesh &m, DevFieldsMixed &f, DevSca
     sched += round_up(pe * sizeof(uint16_t), 256);
     f.rep = reinterpret_cast<uint16_t *>(sched);
     sched += round_up(pe * sizeof(uint16_t), 256);
+    f.eos_blocks = reinterpret_cast<int32_t *>(sched);
+    sched += round_up(m.eos_blocks.size() * sizeof(int32_t), 256);
     *s = reinterpret_cast<DevScalars *>(sched);
 }
 
@@ -203,6 +211,7 @@ hipError_t upload_mesh_mixed(const HostMesh &m, const DevFieldsMixed &f,
     v.eos_order = f.eos_order;
     v.eos_rep = f.eos_rep;
     v.rep = f.rep;
+    v.eos_blocks = f.eos_blocks;
     return upload_mesh(m, v, stream);
 }
 

@@ -32,7 +32,64 @@ int check_cube(const lulesh::MeshConfig &c)
         }
         seen[size_t(el)] = 1;
     }
-    return 0;
+    const size_t g = (size_t(ne) + lulesh::kEosBlockElems - 1) / lulesh::kEosBlockElems;
+    std::vector<char> seen_block(g, 0);
+    for (int32_t b : m.eos_blocks) {
+        if (b < 0 || size_t(b) >= g || seen_block[size_t(b)]) {
+            std::printf("cube nx=%d: eos_blocks is not a permutation\n", c.nx);
+            return 1;
+        }
+        seen_block[size_t(b)] = 1;
+    }
+    return m.eos_blocks.size() == g ? 0 : 1;
+}
+
+// The Q/EOS block order on synthetic repetition counts: g blocks, the blocks
+// in `costly` with 20 repetitions per element (20 passes), the rest with one.
+// Checks: a permutation; every costly block in the first 3/4 of its XCD's
+// slots (slot t goes to XCD t % 8); per XCD, the costly blocks' slot numbers
+// spread over the 32 residues mod 32 (the XCD's CUs) within 1.5x the mean
+// + 2; the other blocks in mesh order, each XCD one contiguous range.
+int check_block_order(size_t g, const std::vector<size_t> &costly, const char *name)
+{
+    std::vector<uint16_t> rep(g * lulesh::kEosBlockElems, 1);
+    std::vector<char> is_costly(g, 0);
+    for (size_t b : costly) {
+        is_costly[b] = 1;
+        for (int i = 0; i < lulesh::kEosBlockElems; ++i) rep[b * lulesh::kEosBlockElems + i] = 20;
+    }
+    const std::vector<int32_t> order = lulesh::eos_block_order(rep);
+    std::vector<int64_t> rank(g, -1);  // place among the cheap blocks, in mesh order
+    for (size_t b = 0, k = 0; b < g; ++b) {
+        if (!is_costly[b]) rank[b] = int64_t(k++);
+    }
+    std::vector<char> seen(g, 0);
+    int bad = order.size() != g;
+    for (int32_t b : order) {
+        bad |= (b < 0 || size_t(b) >= g || seen[size_t(b)]);
+        if (!bad) seen[size_t(b)] = 1;
+    }
+    for (size_t x = 0; x < 8 && !bad; ++x) {
+        const size_t n_x = (g - x + 7) / 8;
+        size_t bins[32] = {}, h_x = 0;
+        int64_t last = -1;
+        for (size_t r = 0; r < n_x; ++r) {
+            const int32_t b = order[8 * r + x];
+            if (is_costly[size_t(b)]) {
+                bad |= r >= (3 * n_x + 3) / 4;
+                bins[r % 32]++;
+                h_x++;
+            } else {
+                bad |= (last >= 0 && rank[size_t(b)] != last + 1);  // contiguous, ascending
+                last = rank[size_t(b)];
+            }
+        }
+        for (size_t c = 0; c < 32; ++c) {
+            bad |= double(bins[c]) > 1.5 * double(h_x) / 32.0 + 2.0;
+        }
+    }
+    if (bad) std::printf("block order %s: failed\n", name);
+    return bad;
 }
 
 }  // namespace
@@ -53,6 +110,19 @@ int main(int argc, char **argv)
                 bad |= check_cube(c);
             }
         }
+    }
+    {
+        // 4096 blocks, 768 costly: 96 of each XCD's 512 slots, whose first
+        // 3/4 (384) at a regular spacing would be exactly every 4th slot
+        std::vector<size_t> costly;
+        for (size_t b = 0; b < 768; ++b) costly.push_back((b * 4099) % 4096);
+        bad |= check_block_order(4096, costly, "period 4");
+        costly.clear();
+        for (size_t b = 0; b < 1000; ++b) costly.push_back((b * 7919) % 3371);
+        bad |= check_block_order(3371, costly, "ragged");
+        bad |= check_block_order(1000, {}, "even");
+        bad |= check_block_order(5, {1, 2, 3, 4, 0}, "all costly");
+        bad |= check_block_order(3, {2}, "tiny");
     }
     for (int nx : {0, 601, -3}) {
         lulesh::MeshConfig c;

@@ -99,3 +99,24 @@ def test_mixed_volume_error(lulesh_gpu):
     d.set("x", x.reshape(-1))
     rc, _ = d.step(1, raise_on_error=False)
     assert rc == lulesh_gpu.VOLUME_ERROR
+
+
+def test_mixed_schedule_does_not_change_results(lulesh_gpu):
+    """The mixed form's Q/EOS in mesh order (bands 0, the default, with the
+    EOS repetitions shared over each wave) and in the region-sorted schedule
+    (bands > 0), under several sweep shapes: each element's arithmetic is the
+    same, so every choice gives the same bits, over 11 regions with their
+    repetition counts (cost 2)."""
+    runs = {}
+    for shape, bands in (((32, 8, 12, 3), 0), ((32, 8, 12, 3), 3), ((16, 16, 5, 3), 1),
+                         ((32, 16, 9, 2), 7), ((16, 8, 4, 3), 0)):
+        d = lulesh_gpu.Domain(26, ic_mode=2, num_reg=11, cost=2, max_cycles=60, mixed=True)
+        d.set_sweep_shape(*shape)
+        d.set_eos_bands(bands)
+        assert d.run() == (0, 60)
+        runs[(shape, bands)] = {f: d.get(f) for f in ("x", "xd", "e", "p", "q", "v", "ss")}
+        d.close()
+    ref = next(iter(runs.values()))
+    for key, r in runs.items():
+        for f in r:
+            assert np.array_equal(r[f].view(np.uint8), ref[f].view(np.uint8)), (key, f)
