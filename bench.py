@@ -201,21 +201,29 @@ def bench_batched(lulesh_amd, plain_value: float) -> dict:
     }
 
 
-# The unstructured step's kernels (lulesh_create_mesh): corner forces
-# (nodelist 32, p,q,v,volo,ss,elemMass 48, write stress and hourglass
-# corner forces 384 per element; x..zd 48 per node), the node gather
-# (corner list 32 + the corner forces 384 per element; corner range 4,
-# mass 8, symmetry byte 1, read xd,x 48 and write them 48 per node), the
-# kinematics with the nodelist (176 + 32) and Q/EOS/dt with the face
-# neighbours, face masks and EOS schedule instead of the region index
-# (264 - 12 + 24 + 4 + 4 + 2).  Sum: 1374 B/element + 253 B/node.
-KERNEL_BYTES_U = {
-    "k_corner_forces_u": (464, 48),
-    "k_nodes_u": (416, 109),
-    "k_kinematics_q": (208, 96),
-    "k_q_eos_dt": (286, 0),
-    "k_time_increment": (0, 0),
-}
+# Algorithmic bytes of the unstructured step's kernels (lulesh_create_mesh)
+# per cycle, from how the force kernel cuts the mesh (Domain.force_blocks):
+# - k_forces_u_blocks: per element the nodelist 32, its corners' slots 32,
+#   p, q, v, volo, ss, elemMass 48; x..zd 48 per node; stress and hourglass
+#   forces 48 per boundary corner written (each padded to two 32-byte
+#   sectors: 64 move); per interior node its corner-list start 4, its local
+#   corner list 2 per corner, mass 8, symmetry byte 1, x..zd written 48;
+# - k_nodes_u: per boundary node its corner range 4, mass 8, symmetry byte
+#   1, x..zd read 48 and written 48; its corners' forces 48 each;
+# - the kinematics with the nodelist (176 + 32 per element, 96 per node) and
+#   Q/EOS/dt with the face neighbours, face masks and EOS schedule instead of
+#   the region index (264 - 12 + 24 + 4 + 4 + 2 per element).
+# With every node a boundary node this is the corner-array scheme's 1374 B
+# per element + 253 B per node.
+def unstructured_bytes(ne: int, nn: int, fb: dict) -> dict:
+    ni, nb, cb = fb["interior_nodes"], fb["boundary_nodes"], fb["boundary_corners"]
+    ci = 8 * ne - cb  # corners of interior nodes
+    return {
+        "k_forces_u_blocks": 112 * ne + 48 * nn + 48 * cb + 61 * ni + 2 * ci,
+        "k_nodes_u": 109 * nb + 48 * cb,
+        "k_kinematics_q": 208 * ne + 96 * nn,
+        "k_q_eos_dt": 286 * ne,
+    }
 
 
 def cube_lists(nx: int) -> dict:
@@ -245,8 +253,9 @@ def cube_lists(nx: int) -> dict:
 
 
 def bench_unstructured(lulesh_amd, nx: int = 150, steps: int = 50, warmup: int = 5) -> dict:
-    """The unstructured step (explicit connectivity, corner forces gathered
-    per node) on the Sedov cube nx^3 given as mesh lists, LULESH 1.0 ICs,
+    """The unstructured step (explicit connectivity, element blocks with
+    on-chip corner forces for their interior nodes) on the Sedov cube nx^3
+    given as mesh lists, LULESH 1.0 ICs,
     beside the structured step on the same cube: zones*cycles/s over
     `steps` cycles and the roofline of each unstructured kernel."""
     ne, nn = nx ** 3, (nx + 1) ** 3
@@ -269,13 +278,15 @@ def bench_unstructured(lulesh_amd, nx: int = 150, steps: int = 50, warmup: int =
             d.set_kernel_timing(True)
             d.step(kt)
             times = d.kernel_times()
+            fb = d.force_blocks()
+            kbytes = unstructured_bytes(ne, nn, fb)
+            out[name]["force_blocks"] = fb
             kern = {}
             for k, (ms, cnt) in times.items():
-                if not cnt or k not in KERNEL_BYTES_U:
+                if not cnt or k not in kbytes:
                     continue
-                be, bn = KERNEL_BYTES_U[k]
                 mean = ms / cnt
-                b = be * ne + bn * nn
+                b = kbytes[k]
                 kern[k] = {"mean_ms": mean, "alg_bytes": b,
                            "achieved_gbs": b / (mean * 1e-3) / 1e9 if b else 0.0,
                            "frac": b / (mean * 1e-3) / 1e9 / HBM_PEAK_GBS if b else 0.0}
@@ -283,9 +294,9 @@ def bench_unstructured(lulesh_amd, nx: int = 150, steps: int = 50, warmup: int =
         d.close()
         print(f"bench: {name} {nx}^3: {out[name]['ms_per_cycle']:.3f} ms/cycle",
               file=sys.stderr, flush=True)
-    b_cycle = 1374 * ne + 253 * nn
-    roof = HBM_PEAK_GBS * 1e9 / b_cycle * ne
     u = out["unstructured"]
+    b_cycle = sum(unstructured_bytes(ne, nn, u["force_blocks"]).values())
+    roof = HBM_PEAK_GBS * 1e9 / b_cycle * ne
     return {
         "value": u["value"], "unit": "zones·cycles/s",
         "workload": f"sedov_{nx}^3 as explicit mesh lists, LULESH 1.0 ICs, cycles "

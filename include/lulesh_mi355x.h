@@ -129,6 +129,15 @@ int lulesh_create_mesh(int num_elem, int num_node, const double *x, const double
                        const lulesh_config *cfg, lulesh_domain **out);
 /* The element holding the initial energy (0 on the cube). */
 int lulesh_origin_element(const lulesh_domain *d);
+/* How an unstructured domain's force kernel cuts the mesh, no counterpart in
+ * the reference: out[0] element blocks (at most 512 elements each, by
+ * recursive coordinate bisection), out[1] interior nodes (all their
+ * elements in one block: summed from on-chip corner forces), out[2]
+ * boundary nodes (the rest: summed from corner forces in device memory),
+ * out[3] element corners of boundary nodes (the corner forces written to
+ * device memory per cycle, stress and hourglass).  LULESH_BAD_ARG on the
+ * cube. */
+int lulesh_force_blocks(const lulesh_domain *d, long long out[4]);
 /* The element corners [num_elem][8] (node indices, bottom face
  * counter-clockwise, then the top face): the nodelist of
  * reference/C/LULESH-OMP.cc:2996-3012 for the cube, the file's lists for a

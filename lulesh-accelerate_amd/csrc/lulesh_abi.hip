@@ -30,14 +30,14 @@ enum KernelId {
     KID_KINEMATICS_Q,
     KID_Q_EOS_DT,
     KID_DT_FINISH,
-    KID_CORNER_FORCES_U,  // unstructured meshes: element corner forces
-    KID_NODES_U,          //   and the node gather and update
+    KID_FORCES_U,         // unstructured meshes: element blocks and their interior nodes
+    KID_NODES_U,          //   and the boundary nodes
     KID_COUNT
 };
 
 const char *const kKernelNames[KID_COUNT] = {
     "k_time_increment", "k_force_sweep", "k_kinematics_q", "k_q_eos_dt",
-    "k_dt_finish", "k_corner_forces_u", "k_nodes_u",
+    "k_dt_finish", "k_forces_u_blocks", "k_nodes_u",
 };
 
 // default launch shape of the force sweep (32 x 8 element columns, at most
@@ -100,7 +100,14 @@ struct lulesh_domain {
     bool fused = false;   // float64 phases with multiply-adds fused (lulesh::fused)
     bool unstructured = false;  // an explicit-connectivity mesh (mesh), f
     DevMeshU mesh{};
+    DevForceBlocksU fblocks{};  // its element blocks
+    int64_t boundary_corners = 0;
+    // An unstructured mesh is held renumbered (renumber_mesh_u): the new
+    // number of each element and node of the mesh as given, which the field
+    // copies map, and the nodelist as given ([8][num_elem]).
+    std::vector<int32_t> elem_new, node_new, nodelist_given;
     int origin = 0;             // the element holding the initial energy
+    int origin_dev = 0;         //   and its number on the device
     std::vector<void *> bufs;   // device allocations of an unstructured domain
     int eos_bands = 0;   // 0: mesh-order Q/EOS; else the interleave of eos_chunk
     const MlpFrags *learned = nullptr;  // learned-Q network, if set
@@ -196,7 +203,8 @@ int launch(lulesh_domain *d, int kid, F &&body)
 int queue_cycle_u(lulesh_domain *d)
 {
     const unsigned ge = unsigned((d->num_elem + kBlock - 1) / kBlock);
-    const unsigned gn = unsigned((d->num_node + kBlock - 1) / kBlock);
+    const DevForceBlocksU &b = d->fblocks;  // a renumbered mesh
+    const unsigned gb = unsigned((b.num_boundary + kBlock - 1) / kBlock);
     hipStream_t st = d->stream;
     DevScalars *s = d->s;
     const DevFields &f = d->f;
@@ -207,13 +215,15 @@ int queue_cycle_u(lulesh_domain *d)
              hipLaunchKernelGGL(k_time_increment, dim3(1), dim3(64), 0, st, s, p.ic_mode);
          })))
         return rc;
-    if ((rc = launch(d, KID_CORNER_FORCES_U, [&] {
-             hipLaunchKernelGGL(k_corner_forces_u, dim3(ge), dim3(kBlock), 0, st, f, m, s, p);
+    if (b.num_blocks > 0 && (rc = launch(d, KID_FORCES_U, [&] {
+             hipLaunchKernelGGL(k_forces_u_blocks, dim3(unsigned(b.num_blocks)),
+                                dim3(kForceBlockElems), 0, st, f, m, b, s, p);
          })))
         return rc;
-    if ((rc = launch(d, KID_NODES_U, [&] {
-             hipLaunchKernelGGL(k_nodes_u, dim3(gn), dim3(kBlock), 0, st, f, m, s, p);
-         })))
+    if (gb > 0 && (rc = launch(d, KID_NODES_U, [&] {
+                       hipLaunchKernelGGL(k_nodes_u, dim3(gb), dim3(kBlock), 0, st, f, m, s, p,
+                                          b.first_boundary, b.num_boundary);
+                   })))
         return rc;
     if ((rc = launch(d, KID_KINEMATICS_Q, [&] {
              hipLaunchKernelGGL(k_kinematics_q_u, dim3(ge), dim3(kBlock), 0, st, f, m, s, p);
@@ -320,25 +330,39 @@ int collect_timings(lulesh_domain *d)
 struct FieldRef {
     void *ptr;
     int64_t n;
-    bool f32;  // stored as float32 (mixed precision)
+    bool f32;                  // stored as float32 (mixed precision)
+    const int32_t *renumber;   // device place of each value, or null (in order)
 };
 
 // par: the copy of the node arrays that holds the current state
 FieldRef field_ref(lulesh_domain *d, int id, int par)
 {
-    FieldRef r{nullptr, 0, false};
+    FieldRef r{nullptr, 0, false, nullptr};
     if (d->mixed) {
         r.ptr = field_pointer_mixed(d->fm, id, par, &r.n, &r.f32);
     } else {
         r.ptr = field_pointer(d->f, id, par, &r.n);
     }
+    if (d->unstructured && r.ptr) {
+        r.renumber = (id < LULESH_F_E) ? d->node_new.data() : d->elem_new.data();
+    }
     return r;
 }
 
-// host float64 <-> device field (converted when the field is float32);
-// synchronous
+// host float64 <-> device field (converted when the field is float32, put
+// in the device's order when the mesh is renumbered); synchronous
 int copy_in(lulesh_domain *d, const FieldRef &r, const double *src)
 {
+    if (r.renumber) {
+        std::vector<double> tmp(static_cast<size_t>(r.n));
+        for (int64_t i = 0; i < r.n; ++i) {
+            tmp[size_t(r.renumber[i])] = src[i];
+        }
+        HIP_TRY(d, hipMemcpyAsync(r.ptr, tmp.data(), tmp.size() * sizeof(double),
+                                  hipMemcpyHostToDevice, d->stream));
+        HIP_TRY(d, hipStreamSynchronize(d->stream));
+        return LULESH_OK;
+    }
     if (!r.f32) {
         HIP_TRY(d, hipMemcpyAsync(r.ptr, src, size_t(r.n) * sizeof(double),
                                   hipMemcpyHostToDevice, d->stream));
@@ -357,6 +381,16 @@ int copy_in(lulesh_domain *d, const FieldRef &r, const double *src)
 
 int copy_out(lulesh_domain *d, double *dst, const FieldRef &r)
 {
+    if (r.renumber) {
+        std::vector<double> tmp(static_cast<size_t>(r.n));
+        HIP_TRY(d, hipMemcpyAsync(tmp.data(), r.ptr, tmp.size() * sizeof(double),
+                                  hipMemcpyDeviceToHost, d->stream));
+        HIP_TRY(d, hipStreamSynchronize(d->stream));
+        for (int64_t i = 0; i < r.n; ++i) {
+            dst[i] = tmp[size_t(r.renumber[i])];
+        }
+        return LULESH_OK;
+    }
     if (!r.f32) {
         HIP_TRY(d, hipMemcpyAsync(dst, r.ptr, size_t(r.n) * sizeof(double),
                                   hipMemcpyDeviceToHost, d->stream));
@@ -520,7 +554,9 @@ int upload_vec(lulesh_domain *d, const std::vector<T> &v, T **out, size_t min_co
     return LULESH_OK;
 }
 
-int build_device_domain_u(lulesh_domain *d, const HostMeshU &m)
+// m and fb are the sources of asynchronous copies: they must outlive the
+// domain's stream work (the caller destroys the domain on failure first)
+int build_device_domain_u(lulesh_domain *d, const HostMeshU &m, const ForceBlocksU &fb)
 {
     const size_t ne = size_t(m.num_elem), nn = size_t(m.num_node);
     DevFields &f = d->f;
@@ -567,15 +603,38 @@ int build_device_domain_u(lulesh_domain *d, const HostMeshU &m)
     g.elem_bc = a;
     if ((rc = upload_vec(d, m.corner_start, &a))) return rc;
     g.corner_start = a;
-    if ((rc = upload_vec(d, m.corner_list, &a))) return rc;
-    g.corner_list = a;
     uint8_t *sy = nullptr;
     if ((rc = upload_vec(d, m.symm, &sy))) return rc;
     g.symm = sy;
-    const std::vector<double> corners(8 * ne, 0.0);
-    for (int t = 0; t < 3; ++t) {
-        if ((rc = upload_vec(d, corners, &g.fs[t])) || (rc = upload_vec(d, corners, &g.fh[t])))
+    // the element blocks of the force kernel, and the slots of the boundary
+    // nodes' corner forces
+    DevForceBlocksU &b = d->fblocks;
+    b.num_blocks = fb.num_blocks;
+    b.first_boundary = fb.node_start.back();
+    b.num_boundary = int(nn) - b.first_boundary;
+    g.corner_base = m.corner_start[size_t(b.first_boundary)];
+    d->boundary_corners = int64_t(m.corner_start[nn]) - g.corner_base;
+    {
+        const std::vector<int32_t> slots = boundary_corner_slots(m, b.first_boundary);
+        if ((rc = upload_vec(d, slots, &a))) return rc;
+        g.corner_slot = a;
+        const std::vector<double> none;
+        // at least one slot: a node with no corner reads slot 0 (and adds 0.0)
+        const size_t n4 = 4 * size_t(std::max<int64_t>(d->boundary_corners, 1));
+        if ((rc = upload_vec(d, none, &g.bs, n4)) || (rc = upload_vec(d, none, &g.bh, n4)))
             return rc;
+    }
+    {
+        int32_t *p = nullptr;
+        if ((rc = upload_vec(d, fb.elem_start, &p))) return rc;
+        b.elem_start = p;
+        if ((rc = upload_vec(d, fb.node_start, &p))) return rc;
+        b.node_start = p;
+        if ((rc = upload_vec(d, fb.ncorner_start, &p))) return rc;
+        b.ncorner_start = p;
+        uint16_t *u16 = nullptr;
+        if ((rc = upload_vec(d, fb.ncorners, &u16))) return rc;
+        b.ncorners = u16;
     }
     DevScalars *sc = nullptr;
     const std::vector<DevScalars> one(1);
@@ -602,9 +661,12 @@ int create_domain_u(const MeshLists &lists, const lulesh_config *cfg, lulesh_dom
     if (!(c.stoptime > 0.0) || c.max_cycles < 0 || c.ic_mode != LULESH_IC_1_0) {
         return LULESH_BAD_ARG;
     }
-    HostMeshU m;
+    HostMeshU given, m;
+    ForceBlocksU fb;
     try {
-        m = build_mesh_u(lists);
+        given = build_mesh_u(lists);
+        fb = build_force_blocks(given);
+        m = renumber_mesh_u(given, fb);
     } catch (const std::exception &) {
         return LULESH_BAD_ARG;
     }
@@ -619,7 +681,11 @@ int create_domain_u(const MeshLists &lists, const lulesh_config *cfg, lulesh_dom
     d->num_node = m.num_node;
     d->num_reg = 1;
     d->reg_num_list.assign(size_t(m.num_elem), 1);
-    d->origin = m.origin;
+    d->origin = given.origin;
+    d->origin_dev = m.origin;
+    d->elem_new = fb.elem_new;
+    d->node_new = fb.node_new;
+    d->nodelist_given = std::move(given.nodelist);
     d->prm = make_params(LULESH_IC_1_0);
     if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) {
         delete d;
@@ -630,7 +696,7 @@ int create_domain_u(const MeshLists &lists, const lulesh_config *cfg, lulesh_dom
         lulesh_destroy(d);
         return LULESH_HIP_ERROR;
     }
-    const int rc = build_device_domain_u(d, m);
+    const int rc = build_device_domain_u(d, m, fb);
     if (rc != LULESH_OK) {
         lulesh_destroy(d);
         return rc;
@@ -707,13 +773,22 @@ int lulesh_get_nodelist(lulesh_domain *d, int *nodelist)
         }
         return LULESH_OK;
     }
-    std::vector<int32_t> cm(size_t(8 * ne));  // corner-major on the device
-    HIP_TRY(d, hipMemcpyAsync(cm.data(), d->mesh.nodelist, cm.size() * sizeof(int32_t),
-                              hipMemcpyDeviceToHost, d->stream));
-    HIP_TRY(d, hipStreamSynchronize(d->stream));
+    const std::vector<int32_t> &cm = d->nodelist_given;  // corner-major
     for (int64_t el = 0; el < ne; ++el) {
         for (int q = 0; q < 8; ++q) nodelist[el * 8 + q] = cm[size_t(q * ne + el)];
     }
+    return LULESH_OK;
+}
+
+int lulesh_force_blocks(const lulesh_domain *d, long long out[4])
+{
+    if (!d || !out || !d->unstructured) {
+        return LULESH_BAD_ARG;
+    }
+    out[0] = d->fblocks.num_blocks;
+    out[1] = d->fblocks.first_boundary;
+    out[2] = d->fblocks.num_boundary;
+    out[3] = d->boundary_corners;
     return LULESH_OK;
 }
 
@@ -881,8 +956,8 @@ int lulesh_get_stats(lulesh_domain *d, lulesh_stats *out)
     int rc = read_scalars(d, &s);
     if (rc) return rc;
     double e0 = 0.0;
-    HIP_TRY(d, hipMemcpy(&e0, energy_ptr(d) + d->origin, sizeof(double),
-                         hipMemcpyDeviceToHost));
+    const int origin = d->unstructured ? d->origin_dev : d->origin;
+    HIP_TRY(d, hipMemcpy(&e0, energy_ptr(d) + origin, sizeof(double), hipMemcpyDeviceToHost));
     out->time = s.time;
     out->deltatime = s.deltatime;
     out->dtcourant = s.dtcourant;

@@ -25,6 +25,7 @@
 // LDS (one global read per node per block) and the advanced nodes are
 // written to the other copy.
 #include "lulesh_kernels.hpp"
+#include "lulesh_mesh.hpp"
 
 #include "lulesh_elem.hpp"
 
@@ -532,84 +533,99 @@ hipError_t launch_shape_batch(int nz, int blocks, const BatchMember *members,
 
 // ---------------------------------------------------------------------------
 // Unstructured meshes.  With an arbitrary connectivity a node's elements are
-// not a fixed stencil, so the element phase writes its 8 corner forces
-// (stress and hourglass apart) and the node phase gathers each node's
-// corners in ascending element order from a corner list: the reference's
-// own scheme (reference/C/LULESH-OMP.cc:853-931, :1475-1496, :209-268).  The
-// element arithmetic is the sweep's, expression for expression.
+// not a fixed stencil.  The reference's scheme (reference/C/
+// LULESH-OMP.cc:853-931, :1475-1496, :209-268) writes every element's 8
+// corner forces (stress and hourglass apart) and gathers each node's corners
+// in ascending element order from a corner list.  Here the elements are cut
+// into blocks (ForceBlocksU, lulesh_mesh.hpp; the mesh is renumbered so that
+// each block's elements and interior nodes are consecutive): a block stages
+// its elements' corner forces in LDS and sums and moves the nodes whose
+// elements all lie in it; only the corners of the other nodes go to device
+// memory, in those nodes' corner-list order, and k_nodes_u sums and moves
+// them.  Every node's sums keep the corner list's order, so the bits are the
+// reference scheme's.  The element arithmetic is the sweep's, expression for
+// expression.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_corner_forces_u(DevFields f, DevMeshU m,
-                                                         DevScalars *s, PhysParams prm)
+namespace {
+
+// one element's inputs: corner positions and velocities, p, q, v, volo, ss
+// and mass
+struct ElemU {
+    double x[8], y[8], z[8];
+    double vel[3][8];
+    double p, q, v, volo, ss, emass;
+};
+
+// everything the element reads is loaded before any store, which the
+// compiler must assume may alias it: one wait for memory, not three
+__device__ __forceinline__ void load_elem_u(const DevFields &f, const DevMeshU &m, int par,
+                                            int el, ElemU &E)
 {
-    if (s->active == 0 || s->error != 0) {
-        return;
-    }
-    const int el = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
     const int ne = f.num_elem;
-    if (el >= ne) {
-        return;
-    }
-    const int par = s->cur;
     int nodes[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         nodes[c] = m.nodelist[c * ne + el];
     }
-    double x[8], y[8], z[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-        x[c] = f.x[par][nodes[c]];
-        y[c] = f.y[par][nodes[c]];
-        z[c] = f.z[par][nodes[c]];
+        E.x[c] = f.x[par][nodes[c]];
+        E.y[c] = f.y[par][nodes[c]];
+        E.z[c] = f.z[par][nodes[c]];
     }
-    // everything the element reads is loaded before its first store, which
-    // the compiler must assume may alias it: one wait for memory, not three
-    const double p_el = f.p[el], q_el = f.q[el];
-    const double vv = f.v[el], volo = f.volo[el], ss = f.ss[el], emass = f.elem_mass[el];
-    double vel[3][8];
+    E.p = f.p[el];
+    E.q = f.q[el];
+    E.v = f.v[el];
+    E.volo = f.volo[el];
+    E.ss = f.ss[el];
+    E.emass = f.elem_mass[el];
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-        vel[0][c] = f.xd[par][nodes[c]];
-        vel[1][c] = f.yd[par][nodes[c]];
-        vel[2][c] = f.zd[par][nodes[c]];
+        E.vel[0][c] = f.xd[par][nodes[c]];
+        E.vel[1][c] = f.yd[par][nodes[c]];
+        E.vel[2][c] = f.zd[par][nodes[c]];
     }
-    {
-        const double sig = -p_el - q_el;
-        const double det = jacobian_volume(x, y, z);
-        // VolumeError on the Jacobian volume, then on v, which the reference
-        // tests for every element before it looks at hgcoef (:1579-1584,
-        // :1540-1543)
-        if (det <= 0.0 || vv <= 0.0) {
-            atomicCAS(&s->error, 0, -1);
-        }
-        double pf[3][8];
-        node_normals(pf[0], pf[1], pf[2], x, y, z);
+}
+
+// stress corner forces (IntegrateStressForElems); returns whether the
+// element raises VolumeError: on the Jacobian volume, then on v, which the
+// reference tests for every element before it looks at hgcoef (:1579-1584,
+// :1540-1543)
+__device__ __forceinline__ bool stress_corners_u(const ElemU &E, double fs[3][8])
+{
+    const double sig = -E.p - E.q;
+    const double det = jacobian_volume(E.x, E.y, E.z);
+    double pf[3][8];
+    node_normals(pf[0], pf[1], pf[2], E.x, E.y, E.z);
 #pragma unroll
-        for (int t = 0; t < 3; ++t) {
+    for (int t = 0; t < 3; ++t) {
 #pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                m.fs[t][c * ne + el] = -(sig * pf[t][c]);
-            }
+        for (int c = 0; c < 8; ++c) {
+            fs[t][c] = -(sig * pf[t][c]);
         }
     }
-    if (!(prm.hgcoef > 0.0)) {
-        return;
-    }
-    const double determ = volo * vv;
+    return det <= 0.0 || E.v <= 0.0;
+}
+
+// hourglass corner forces (CalcFBHourglassForceForElems)
+__device__ __forceinline__ void hourglass_corners_u(const ElemU &E, const PhysParams &prm,
+                                                    double fh[3][8])
+{
+    const double determ = E.volo * E.v;
     const double volinv = 1.0 / determ;
     double hourgam[8][4];
     {
         double hm[4][3];
 #pragma unroll
         for (int i1 = 0; i1 < 4; ++i1) {
-            double hmx = x[0] * fb_gamma(i1, 0);
-            double hmy = y[0] * fb_gamma(i1, 0);
-            double hmz = z[0] * fb_gamma(i1, 0);
+            double hmx = E.x[0] * fb_gamma(i1, 0);
+            double hmy = E.y[0] * fb_gamma(i1, 0);
+            double hmz = E.z[0] * fb_gamma(i1, 0);
 #pragma unroll
             for (int n = 1; n < 8; ++n) {
-                hmx = hmx + x[n] * fb_gamma(i1, n);
-                hmy = hmy + y[n] * fb_gamma(i1, n);
-                hmz = hmz + z[n] * fb_gamma(i1, n);
+                hmx = hmx + E.x[n] * fb_gamma(i1, n);
+                hmy = hmy + E.y[n] * fb_gamma(i1, n);
+                hmz = hmz + E.z[n] * fb_gamma(i1, n);
             }
             hm[i1][0] = hmx;
             hm[i1][1] = hmy;
@@ -619,7 +635,7 @@ __global__ __launch_bounds__(256) void k_corner_forces_u(DevFields f, DevMeshU m
         for (int n = 0; n < 8; ++n) {
             const int *st = kStencil[n];
             double dvx, dvy, dvz;
-            volu_der(x, y, z, st[0], st[1], st[2], st[3], st[4], st[5], dvx, dvy, dvz);
+            volu_der(E.x, E.y, E.z, st[0], st[1], st[2], st[3], st[4], st[5], dvx, dvy, dvz);
 #pragma unroll
             for (int i1 = 0; i1 < 4; ++i1) {
                 hourgam[n][i1] = fb_gamma(i1, n) -
@@ -628,10 +644,10 @@ __global__ __launch_bounds__(256) void k_corner_forces_u(DevFields f, DevMeshU m
         }
     }
     const double volume13 = lulesh_cbrt(determ);
-    const double coefficient = -prm.hgcoef * 0.01 * ss * emass / volume13;
+    const double coefficient = -prm.hgcoef * 0.01 * E.ss * E.emass / volume13;
 #pragma unroll
     for (int t = 0; t < 3; ++t) {
-        const double *u = vel[t];
+        const double *u = E.vel[t];
         double h[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -641,75 +657,18 @@ __global__ __launch_bounds__(256) void k_corner_forces_u(DevFields f, DevMeshU m
         }
 #pragma unroll
         for (int n = 0; n < 8; ++n) {
-            m.fh[t][n * ne + el] =
-                coefficient * (hourgam[n][0] * h[0] + hourgam[n][1] * h[1] +
-                               hourgam[n][2] * h[2] + hourgam[n][3] * h[3]);
+            fh[t][n] = coefficient * (hourgam[n][0] * h[0] + hourgam[n][1] * h[1] +
+                                      hourgam[n][2] * h[2] + hourgam[n][3] * h[3]);
         }
     }
 }
 
-// gather, acceleration, symmetry planes, velocity and position of each node
-// (:905-926, :1475-1496, :1613-1683); positions and velocities are updated
-// in place (the corner forces are complete before this kernel runs)
-__global__ __launch_bounds__(256) void k_nodes_u(DevFields f, DevMeshU m, DevScalars *s,
-                                                 PhysParams prm)
+// acceleration, symmetry planes, velocity and position of node n from its
+// force (:1613-1683), in place
+__device__ __forceinline__ void move_node_u(const DevFields &f, const DevMeshU &m, int par,
+                                            double dt, const PhysParams &prm, int n, double fx,
+                                            double fy, double fz)
 {
-    if (s->active == 0 || s->error != 0) {
-        return;
-    }
-    const int n = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
-    if (n >= f.num_node) {
-        return;
-    }
-    const int par = s->cur;
-    const double dt = s->deltatime;
-    const int b = m.corner_start[n], e = m.corner_start[n + 1];
-    const int cnt = e - b;
-    // The first eight corners (all of them on a hexahedral mesh where at most
-    // eight elements meet) are loaded together, without a branch per corner:
-    // an absent one reads corner-list entry 0 and adds +0.0 instead, which
-    // leaves the sum's bits as skipping it does (a sum started from +0.0 is
-    // never -0.0).  Further corners follow in list order.
-    int ci[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        ci[c] = m.corner_list[(c < cnt) ? b + c : 0];
-    }
-    auto gather = [&](const double *const *a, double &gx, double &gy, double &gz) {
-        double v[3][8];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-#pragma unroll
-            for (int t = 0; t < 3; ++t) {
-                v[t][c] = a[t][ci[c]];
-            }
-        }
-        double ax = 0.0, ay = 0.0, az = 0.0;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            ax += (c < cnt) ? v[0][c] : 0.0;
-            ay += (c < cnt) ? v[1][c] : 0.0;
-            az += (c < cnt) ? v[2][c] : 0.0;
-        }
-        for (int i = b + 8; i < e; ++i) {
-            const int c = m.corner_list[i];
-            ax += a[0][c];
-            ay += a[1][c];
-            az += a[2][c];
-        }
-        gx = ax;
-        gy = ay;
-        gz = az;
-    };
-    double fx, fy, fz;
-    gather(m.fs, fx, fy, fz);
-    if (prm.hgcoef > 0.0) {
-        double hx, hy, hz;
-        gather(m.fh, hx, hy, hz);
-        fx += hx;
-        fy += hy;
-        fz += hz;
-    }
     const double mass = f.nodal_mass[n];
     double xdd = fx / mass;
     double ydd = fy / mass;
@@ -730,6 +689,190 @@ __global__ __launch_bounds__(256) void k_nodes_u(DevFields f, DevMeshU m, DevSca
     f.x[par][n] = f.x[par][n] + xd * dt;
     f.y[par][n] = f.y[par][n] + yd * dt;
     f.z[par][n] = f.z[par][n] + zd * dt;
+}
+
+// one boundary corner's force into its slot: x, y, z and a zero, the whole
+// 32-byte sector in two 16-byte stores (as a 64-byte line holding the stress
+// and the hourglass force of the corner, 3% slower here and 10% slower to
+// gather)
+__device__ __forceinline__ void store_corner(double *a, int slot, double x, double y, double z)
+{
+    double2 *p = reinterpret_cast<double2 *>(a + 4 * size_t(slot));
+    p[0] = make_double2(x, y);
+    p[1] = make_double2(z, 0.0);
+}
+
+}  // namespace
+
+// One block of elements (b.elem_start[blockIdx.x] ..., consecutive), one
+// element per thread, then the block's interior nodes (consecutive), one per
+// thread.  The stress corners and then the hourglass corners are staged in
+// LDS (96 KB, one kind at a time); a corner whose node is a boundary node
+// (m.corner_slot >= 0) is also written to its slot, one whole 32-byte sector
+// (a partial sector written per corner cost the kernel 70%).  Each interior
+// node sums its stress corners from 0.0 in corner-list order, then its
+// hourglass corners from 0.0, adds the two and moves.  The block's interior
+// nodes are read by no other block's elements, so they can move while other
+// blocks still compute.
+__global__ __launch_bounds__(kForceBlockElems) void k_forces_u_blocks(DevFields f, DevMeshU m,
+                                                                      DevForceBlocksU b,
+                                                                      DevScalars *s,
+                                                                      PhysParams prm)
+{
+    constexpr int kN = kForceBlockElems;
+    __shared__ double corner[3][8][kN];
+    __shared__ int skip;
+    // other blocks may raise an error meanwhile: read the flags once per
+    // block, so that every thread takes the same path to the barriers
+    if (threadIdx.x == 0) {
+        skip = (s->active == 0 || s->error != 0) ? 1 : 0;
+    }
+    __syncthreads();
+    if (skip) {
+        return;
+    }
+    const int blk = int(blockIdx.x), l = int(threadIdx.x);
+    const int ne = f.num_elem;
+    const int par = s->cur;
+    const double dt = s->deltatime;
+    const int e0 = b.elem_start[blk];
+    const int el = e0 + l;
+    const bool has = el < b.elem_start[blk + 1];
+    const int node = b.node_start[blk] + l;
+    const bool own = node < b.node_start[blk + 1];
+    int slot[8];
+    ElemU E;
+    if (has) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            slot[c] = m.corner_slot[c * ne + el];
+        }
+        load_elem_u(f, m, par, el, E);
+    }
+    int cb = 0, ce = 0;
+    if (own) {
+        cb = b.ncorner_start[node];
+        ce = b.ncorner_start[node + 1];
+    }
+    if (has) {
+        double fs[3][8];
+        if (stress_corners_u(E, fs)) {
+            atomicCAS(&s->error, 0, -1);
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                corner[t][c][l] = fs[t][c];
+            }
+            if (slot[c] >= 0) {
+                store_corner(m.bs, slot[c], fs[0][c], fs[1][c], fs[2][c]);
+            }
+        }
+    }
+    __syncthreads();
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int i = cb; i < ce; ++i) {
+        const int k = b.ncorners[i];
+        fx += corner[0][k & 7][k >> 3];
+        fy += corner[1][k & 7][k >> 3];
+        fz += corner[2][k & 7][k >> 3];
+    }
+    if (prm.hgcoef > 0.0) {
+        __syncthreads();  // the stress corners are summed
+        if (has) {
+            double fh[3][8];
+            hourglass_corners_u(E, prm, fh);
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+#pragma unroll
+                for (int t = 0; t < 3; ++t) {
+                    corner[t][c][l] = fh[t][c];
+                }
+                if (slot[c] >= 0) {
+                    store_corner(m.bh, slot[c], fh[0][c], fh[1][c], fh[2][c]);
+                }
+            }
+        }
+        __syncthreads();
+        double hx = 0.0, hy = 0.0, hz = 0.0;
+        for (int i = cb; i < ce; ++i) {
+            const int k = b.ncorners[i];
+            hx += corner[0][k & 7][k >> 3];
+            hy += corner[1][k & 7][k >> 3];
+            hz += corner[2][k & 7][k >> 3];
+        }
+        fx += hx;
+        fy += hy;
+        fz += hz;
+    }
+    if (own) {
+        move_node_u(f, m, par, dt, prm, node, fx, fy, fz);
+    }
+}
+
+// The boundary nodes first .. first + count - 1, one per thread: gather of
+// their corners from their slots, acceleration, symmetry planes,
+// velocity and position (:905-926, :1475-1496, :1613-1683), in place (the
+// corner forces are complete before this kernel runs)
+__global__ __launch_bounds__(256) void k_nodes_u(DevFields f, DevMeshU m, DevScalars *s,
+                                                 PhysParams prm, int first, int count)
+{
+    if (s->active == 0 || s->error != 0) {
+        return;
+    }
+    const int i = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
+    if (i >= count) {
+        return;
+    }
+    const int n = first + i;
+    const int par = s->cur;
+    const double dt = s->deltatime;
+    // the node's corners are slots b .. e - 1, consecutive
+    const int b = m.corner_start[n] - m.corner_base, e = m.corner_start[n + 1] - m.corner_base;
+    const int cnt = e - b;
+    // The first eight corners (all of them on a hexahedral mesh where at most
+    // eight elements meet) are loaded together, without a branch per corner:
+    // an absent one reads slot 0 and adds +0.0 instead, which leaves the
+    // sum's bits as skipping it does (a sum started from +0.0 is never
+    // -0.0).  Further corners follow in order.
+    auto gather = [&](const double *a, double &gx, double &gy, double &gz) {
+        double v[3][8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const size_t j = size_t((c < cnt) ? b + c : 0);
+            const double2 *p = reinterpret_cast<const double2 *>(a + 4 * j);
+            const double2 xy = p[0], zw = p[1];
+            v[0][c] = xy.x;
+            v[1][c] = xy.y;
+            v[2][c] = zw.x;
+        }
+        double ax = 0.0, ay = 0.0, az = 0.0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            ax += (c < cnt) ? v[0][c] : 0.0;
+            ay += (c < cnt) ? v[1][c] : 0.0;
+            az += (c < cnt) ? v[2][c] : 0.0;
+        }
+        for (int j = b + 8; j < e; ++j) {
+            ax += a[4 * size_t(j)];
+            ay += a[4 * size_t(j) + 1];
+            az += a[4 * size_t(j) + 2];
+        }
+        gx = ax;
+        gy = ay;
+        gz = az;
+    };
+    double fx, fy, fz;
+    gather(m.bs, fx, fy, fz);
+    if (prm.hgcoef > 0.0) {
+        double hx, hy, hz;
+        gather(m.bh, hx, hy, hz);
+        fx += hx;
+        fy += hy;
+        fz += hz;
+    }
+    move_node_u(f, m, par, dt, prm, n, fx, fy, fz);
 }
 
 // (TX, TY, minimum waves per SIMD the register budget must allow); the

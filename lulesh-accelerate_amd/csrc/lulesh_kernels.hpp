@@ -234,11 +234,15 @@ struct DevMeshU {
     const int32_t *nodelist;      // [8][num_elem]
     const int32_t *nbr;           // [6][num_elem]: xi-, xi+, eta-, eta+, zeta-, zeta+
     const int32_t *elem_bc;       // [num_elem] face masks (kBc*)
-    const int32_t *corner_start;  // [num_node + 1]
-    const int32_t *corner_list;   // corners c * num_elem + el of each node, by element
+    const int32_t *corner_start;  // [num_node + 1]: each node's corners in corner-list order
     const uint8_t *symm;          // [num_node] bit 0 / 1 / 2: x / y / z symmetry plane
-    double *fs[3];                // stress corner forces [8][num_elem], per component
-    double *fh[3];                // hourglass corner forces
+    // The boundary nodes' corner forces, in their corner lists' order: slot
+    // j - corner_base for corner-list entry j, 4 doubles (x, y, z, 0) each,
+    // so that every corner is one whole 32-byte sector.
+    const int32_t *corner_slot;   // [8][num_elem] slot of each corner, -1: interior node
+    double *bs;                   // stress [slot][4]
+    double *bh;                   // hourglass [slot][4]
+    int corner_base;
 };
 
 // element face boundary masks (reference/C/LULESH-OMP.cc:532-554)
@@ -247,8 +251,22 @@ constexpr int kBcXiMSymm = 0x001, kBcXiMFree = 0x002, kBcXiPSymm = 0x004,
               kBcEtaPSymm = 0x040, kBcEtaPFree = 0x080, kBcZetaMSymm = 0x100,
               kBcZetaMFree = 0x200, kBcZetaPSymm = 0x400, kBcZetaPFree = 0x800;
 
-__global__ void k_corner_forces_u(DevFields f, DevMeshU m, DevScalars *s, PhysParams prm);
-__global__ void k_nodes_u(DevFields f, DevMeshU m, DevScalars *s, PhysParams prm);
+// The element blocks of an unstructured mesh in device memory (ForceBlocksU,
+// lulesh_mesh.hpp), in the renumbered mesh: block k is elements
+// [elem_start[k], elem_start[k + 1]) with interior nodes [node_start[k],
+// node_start[k + 1]); the boundary nodes are [first_boundary, num_node).
+struct DevForceBlocksU {
+    const int32_t *elem_start;     // [num_blocks + 1]
+    const int32_t *node_start;     // [num_blocks + 1]
+    const int32_t *ncorner_start;  // [first_boundary + 1], into ncorners
+    const uint16_t *ncorners;      // 8 * (element - elem_start[k]) + corner
+    int num_blocks, first_boundary, num_boundary;
+};
+
+__global__ void k_forces_u_blocks(DevFields f, DevMeshU m, DevForceBlocksU b, DevScalars *s,
+                                  PhysParams prm);
+__global__ void k_nodes_u(DevFields f, DevMeshU m, DevScalars *s, PhysParams prm, int first,
+                          int count);
 __global__ void k_kinematics_q_u(DevFields f, DevMeshU m, DevScalars *s, PhysParams prm);
 __global__ void k_q_eos_dt_u(DevFields f, DevMeshU m, DevScalars *s, PhysParams prm);
 

@@ -11,6 +11,8 @@
 #include <cstdlib>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <utility>
 
 namespace lulesh {
 
@@ -370,6 +372,198 @@ std::vector<int32_t> eos_block_order(const std::vector<uint16_t> &rep)
         }
     }
     return order;
+}
+
+// Recursive coordinate bisection: a range of elements larger than one block
+// is cut at the median centroid along its longest extent, into parts holding
+// whole numbers of blocks (the first ceil(k / 2) of the range's k blocks),
+// so every block but the last of a range is full.  Ties are broken by the
+// element index, so the cut is deterministic.  Blocks come out in
+// depth-first order, which keeps consecutive blocks near each other.
+ForceBlocksU build_force_blocks(const HostMeshU &m, int block_elems)
+{
+    if (block_elems < 1 || block_elems > kForceBlockElems) {
+        throw std::invalid_argument("build_force_blocks: block size out of range");
+    }
+    const size_t ne = size_t(m.num_elem), nn = size_t(m.num_node);
+    const size_t cap = size_t(block_elems);
+    std::vector<double> cen[3];
+    const std::vector<double> *xyz[3] = {&m.x, &m.y, &m.z};
+    for (int a = 0; a < 3; ++a) {
+        cen[a].assign(ne, 0.0);
+        for (size_t el = 0; el < ne; ++el) {
+            double sum = 0.0;
+            for (int c = 0; c < 8; ++c) {
+                sum += (*xyz[a])[size_t(m.nodelist[size_t(c) * ne + el])];
+            }
+            cen[a][el] = sum * 0.125;
+        }
+    }
+    std::vector<int32_t> idx(ne);
+    for (size_t el = 0; el < ne; ++el) idx[el] = int32_t(el);
+    ForceBlocksU b;
+    b.elem_start.push_back(0);
+    std::vector<std::pair<size_t, size_t>> stack;
+    if (ne > 0) stack.emplace_back(0, ne);
+    while (!stack.empty()) {
+        const auto [lo, hi] = stack.back();
+        stack.pop_back();
+        const size_t count = hi - lo;
+        if (count <= cap) {
+            std::sort(idx.begin() + std::ptrdiff_t(lo), idx.begin() + std::ptrdiff_t(hi));
+            b.elem_start.push_back(int32_t(hi));
+            continue;
+        }
+        int axis = 0;
+        double best = -1.0;
+        for (int a = 0; a < 3; ++a) {
+            double mn = cen[a][size_t(idx[lo])], mx = mn;
+            for (size_t i = lo; i < hi; ++i) {
+                mn = std::min(mn, cen[a][size_t(idx[i])]);
+                mx = std::max(mx, cen[a][size_t(idx[i])]);
+            }
+            if (mx - mn > best) {
+                best = mx - mn;
+                axis = a;
+            }
+        }
+        const size_t blocks = (count + cap - 1) / cap;
+        const size_t mid = lo + (blocks + 1) / 2 * cap;
+        const std::vector<double> &key = cen[axis];
+        std::nth_element(idx.begin() + std::ptrdiff_t(lo), idx.begin() + std::ptrdiff_t(mid),
+                         idx.begin() + std::ptrdiff_t(hi), [&](int32_t p, int32_t q) {
+                             const double kp = key[size_t(p)], kq = key[size_t(q)];
+                             return kp < kq || (kp == kq && p < q);
+                         });
+        stack.emplace_back(mid, hi);  // popped second: the lower half first
+        stack.emplace_back(lo, mid);
+    }
+    b.num_blocks = int(b.elem_start.size()) - 1;
+    // new element numbers: block by block, ascending within a block
+    b.elem_new.assign(ne, 0);
+    std::vector<int32_t> block_of(ne);
+    for (int k = 0; k < b.num_blocks; ++k) {
+        for (int32_t i = b.elem_start[size_t(k)]; i < b.elem_start[size_t(k) + 1]; ++i) {
+            b.elem_new[size_t(idx[size_t(i)])] = i;
+            block_of[size_t(idx[size_t(i)])] = k;
+        }
+    }
+    // interior nodes by block, ascending; a node with no element is a
+    // boundary node
+    std::vector<std::vector<int32_t>> interior(size_t(b.num_blocks));
+    std::vector<char> is_boundary(nn, 1);
+    for (size_t n = 0; n < nn; ++n) {
+        const int32_t c0 = m.corner_start[n], c1 = m.corner_start[n + 1];
+        if (c0 == c1) continue;
+        const int k = block_of[size_t(m.corner_list[size_t(c0)]) % ne];
+        bool one = true;
+        for (int32_t i = c0 + 1; i < c1 && one; ++i) {
+            one = block_of[size_t(m.corner_list[size_t(i)]) % ne] == k;
+        }
+        if (one && interior[size_t(k)].size() < cap) {
+            interior[size_t(k)].push_back(int32_t(n));
+            is_boundary[n] = 0;
+        }
+    }
+    // new node numbers: the interior nodes block by block, then the boundary
+    // nodes by the first new number of their elements (near the blocks whose
+    // corner forces they read), ties by their own number
+    b.node_new.assign(nn, 0);
+    b.node_start.push_back(0);
+    b.ncorner_start.push_back(0);
+    int32_t next = 0;
+    for (int k = 0; k < b.num_blocks; ++k) {
+        const int32_t e0 = b.elem_start[size_t(k)];
+        for (int32_t n : interior[size_t(k)]) {
+            b.node_new[size_t(n)] = next++;
+            for (int32_t i = m.corner_start[size_t(n)]; i < m.corner_start[size_t(n) + 1]; ++i) {
+                const size_t cl = size_t(m.corner_list[size_t(i)]);
+                b.ncorners.push_back(
+                    uint16_t(8 * (b.elem_new[cl % ne] - e0) + int32_t(cl / ne)));
+            }
+            b.ncorner_start.push_back(int32_t(b.ncorners.size()));
+        }
+        b.node_start.push_back(next);
+    }
+    std::vector<std::pair<int64_t, int32_t>> bnodes;
+    for (size_t n = 0; n < nn; ++n) {
+        if (!is_boundary[n]) continue;
+        int64_t first = int64_t(ne);
+        for (int32_t i = m.corner_start[n]; i < m.corner_start[n + 1]; ++i) {
+            first = std::min<int64_t>(first, b.elem_new[size_t(m.corner_list[size_t(i)]) % ne]);
+        }
+        bnodes.emplace_back(first, int32_t(n));
+    }
+    std::sort(bnodes.begin(), bnodes.end());
+    for (const auto &bn : bnodes) {
+        b.node_new[size_t(bn.second)] = next++;
+    }
+    return b;
+}
+
+std::vector<int32_t> boundary_corner_slots(const HostMeshU &r, int32_t first_boundary)
+{
+    std::vector<int32_t> slot(size_t(r.num_elem) * 8, -1);
+    const int32_t base = r.corner_start[size_t(first_boundary)];
+    for (int32_t j = base; j < r.corner_start[size_t(r.num_node)]; ++j) {
+        slot[size_t(r.corner_list[size_t(j)])] = j - base;
+    }
+    return slot;
+}
+
+HostMeshU renumber_mesh_u(const HostMeshU &m, const ForceBlocksU &b)
+{
+    const size_t ne = size_t(m.num_elem), nn = size_t(m.num_node);
+    HostMeshU r;
+    r.num_elem = m.num_elem;
+    r.num_node = m.num_node;
+    r.origin = b.elem_new[size_t(m.origin)];
+    auto by_node = [&](const auto &v) {
+        std::remove_cv_t<std::remove_reference_t<decltype(v)>> o(v.size());
+        for (size_t n = 0; n < nn; ++n) o[size_t(b.node_new[n])] = v[n];
+        return o;
+    };
+    auto by_elem = [&](const auto &v) {
+        std::remove_cv_t<std::remove_reference_t<decltype(v)>> o(v.size());
+        for (size_t el = 0; el < ne; ++el) o[size_t(b.elem_new[el])] = v[el];
+        return o;
+    };
+    r.x = by_node(m.x);
+    r.y = by_node(m.y);
+    r.z = by_node(m.z);
+    r.nodal_mass = by_node(m.nodal_mass);
+    r.symm = by_node(m.symm);
+    r.volo = by_elem(m.volo);
+    r.e = by_elem(m.e);
+    r.elem_bc = by_elem(m.elem_bc);
+    r.nodelist.assign(8 * ne, 0);
+    for (size_t c = 0; c < 8; ++c) {
+        for (size_t el = 0; el < ne; ++el) {
+            r.nodelist[c * ne + size_t(b.elem_new[el])] =
+                b.node_new[size_t(m.nodelist[c * ne + el])];
+        }
+    }
+    r.nbr.assign(6 * ne, 0);
+    for (size_t f = 0; f < 6; ++f) {
+        for (size_t el = 0; el < ne; ++el) {
+            r.nbr[f * ne + size_t(b.elem_new[el])] = b.elem_new[size_t(m.nbr[f * ne + el])];
+        }
+    }
+    // each node's corners in their old order, renamed
+    r.corner_start.assign(nn + 1, 0);
+    for (size_t n = 0; n < nn; ++n) {
+        r.corner_start[size_t(b.node_new[n]) + 1] = m.corner_start[n + 1] - m.corner_start[n];
+    }
+    for (size_t n = 0; n < nn; ++n) r.corner_start[n + 1] += r.corner_start[n];
+    r.corner_list.assign(m.corner_list.size(), 0);
+    for (size_t n = 0; n < nn; ++n) {
+        int32_t out = r.corner_start[size_t(b.node_new[n])];
+        for (int32_t i = m.corner_start[n]; i < m.corner_start[n + 1]; ++i) {
+            const size_t cl = size_t(m.corner_list[size_t(i)]);
+            r.corner_list[size_t(out++)] = int32_t((cl / ne) * ne + size_t(b.elem_new[cl % ne]));
+        }
+    }
+    return r;
 }
 
 MeshLists read_lmesh(const char *path)

@@ -83,7 +83,7 @@ EXPORTS = (
     "lulesh_q_features", "lulesh_set_learned_q_form", "lulesh_write_domain_arrays",
     "lulesh_write_domain",
     "lulesh_create_lmesh", "lulesh_create_mesh", "lulesh_origin_element",
-    "lulesh_get_nodelist", "lulesh_set_eos_bands",
+    "lulesh_get_nodelist", "lulesh_set_eos_bands", "lulesh_force_blocks",
 )
 
 
@@ -118,6 +118,8 @@ def lib() -> ctypes.CDLL:
     L.lulesh_create_mesh.restype = I
     L.lulesh_origin_element.argtypes = [P]
     L.lulesh_origin_element.restype = I
+    L.lulesh_force_blocks.argtypes = [P, ctypes.POINTER(ctypes.c_longlong)]
+    L.lulesh_force_blocks.restype = I
     L.lulesh_get_nodelist.argtypes = [P, IP]
     L.lulesh_get_nodelist.restype = I
     L.lulesh_set_eos_bands.argtypes = [P, I]
@@ -487,6 +489,16 @@ class Domain:
         ie = float(np.dot(self.get("e"), self.get("elemMass")))
         v2 = self.get("xd") ** 2 + self.get("yd") ** 2 + self.get("zd") ** 2
         return ie + 0.5 * float(np.dot(self.get("nodalMass"), v2))
+
+    def force_blocks(self) -> dict:
+        """How the unstructured force kernel cuts the mesh
+        (lulesh_force_blocks): element blocks, interior nodes (summed on
+        chip), boundary nodes and their element corners (through device
+        memory)."""
+        out = (ctypes.c_longlong * 4)()
+        self._check(lib().lulesh_force_blocks(self.h, out))
+        return {"blocks": out[0], "interior_nodes": out[1], "boundary_nodes": out[2],
+                "boundary_corners": out[3]}
 
     def shock_radius(self) -> dict:
         """Distance from the origin of the centroid of the element with the

@@ -92,6 +92,101 @@ int check_block_order(size_t g, const std::vector<size_t> &costly, const char *n
     return bad;
 }
 
+// The force blocks of an unstructured mesh and the renumbered mesh: the new
+// element and node numbers are permutations; every block holds at most
+// kForceBlockElems elements, ascending in the mesh's numbers; each interior
+// node's elements all lie in its block, and its local corner list names
+// them and their corners in the corner list's order; the renumbered mesh
+// holds the same connectivity, coordinates, masses and corner lists (in the
+// old order) under the new numbers; every corner of a boundary node has the
+// slot of its corner-list entry, and no other corner has one.
+int check_force_blocks(const lulesh::HostMeshU &m, const char *name)
+{
+    const lulesh::ForceBlocksU b = lulesh::build_force_blocks(m);
+    const lulesh::HostMeshU r = lulesh::renumber_mesh_u(m, b);
+    const size_t ne = size_t(m.num_elem), nn = size_t(m.num_node);
+    int bad = 0;
+    std::vector<int32_t> elem_old(ne, -1), node_old(nn, -1);
+    bad |= b.elem_new.size() != ne || b.node_new.size() != nn;
+    for (size_t el = 0; el < ne && !bad; ++el) {
+        const int32_t k = b.elem_new[el];
+        bad |= k < 0 || size_t(k) >= ne || elem_old[size_t(k)] != -1;
+        if (!bad) elem_old[size_t(k)] = int32_t(el);
+    }
+    for (size_t n = 0; n < nn && !bad; ++n) {
+        const int32_t k = b.node_new[n];
+        bad |= k < 0 || size_t(k) >= nn || node_old[size_t(k)] != -1;
+        if (!bad) node_old[size_t(k)] = int32_t(n);
+    }
+    bad |= b.elem_start.size() != size_t(b.num_blocks) + 1 || b.elem_start[0] != 0 ||
+           size_t(b.elem_start.back()) != ne || b.node_start.size() != b.elem_start.size() ||
+           b.node_start[0] != 0;
+    const int32_t first_boundary = bad ? 0 : b.node_start.back();
+    const std::vector<int32_t> slots = lulesh::boundary_corner_slots(r, first_boundary);
+    for (int k = 0; k < b.num_blocks && !bad; ++k) {
+        const int32_t e0 = b.elem_start[size_t(k)], e1 = b.elem_start[size_t(k) + 1];
+        const int32_t n0 = b.node_start[size_t(k)], n1 = b.node_start[size_t(k) + 1];
+        bad |= e1 <= e0 || e1 - e0 > lulesh::kForceBlockElems ||
+               n1 < n0 || n1 - n0 > lulesh::kForceBlockElems;
+        for (int32_t e = e0 + 1; e < e1 && !bad; ++e) {
+            bad |= elem_old[size_t(e)] <= elem_old[size_t(e) - 1];
+        }
+        for (int32_t n = n0; n < n1 && !bad; ++n) {
+            const int32_t on = node_old[size_t(n)];
+            const int32_t c0 = m.corner_start[size_t(on)], c1 = m.corner_start[size_t(on) + 1];
+            bad |= c1 == c0 ||
+                   b.ncorner_start[size_t(n) + 1] - b.ncorner_start[size_t(n)] != c1 - c0;
+            for (int32_t j = 0; j < c1 - c0 && !bad; ++j) {
+                const size_t cl = size_t(m.corner_list[size_t(c0 + j)]);
+                const uint16_t lc = b.ncorners[size_t(b.ncorner_start[size_t(n)] + j)];
+                bad |= e0 + lc / 8 >= e1 || elem_old[size_t(e0 + lc / 8)] != int32_t(cl % ne) ||
+                       lc % 8 != cl / ne;
+            }
+        }
+    }
+    // the renumbered mesh
+    bad |= r.num_elem != m.num_elem || r.num_node != m.num_node ||
+           r.origin != (bad ? 0 : b.elem_new[size_t(m.origin)]);
+    for (size_t n = 0; n < nn && !bad; ++n) {
+        const size_t k = size_t(b.node_new[n]);
+        bad |= r.x[k] != m.x[n] || r.y[k] != m.y[n] || r.z[k] != m.z[n] ||
+               r.nodal_mass[k] != m.nodal_mass[n] || r.symm[k] != m.symm[n] ||
+               r.corner_start[k + 1] - r.corner_start[k] !=
+                   m.corner_start[n + 1] - m.corner_start[n];
+        for (int32_t i = 0; !bad && i < m.corner_start[n + 1] - m.corner_start[n]; ++i) {
+            const size_t cl = size_t(m.corner_list[size_t(m.corner_start[n] + i)]);
+            bad |= size_t(r.corner_list[size_t(r.corner_start[k] + i)]) !=
+                   (cl / ne) * ne + size_t(b.elem_new[cl % ne]);
+        }
+    }
+    for (size_t el = 0; el < ne && !bad; ++el) {
+        const size_t k = size_t(b.elem_new[el]);
+        bad |= r.volo[k] != m.volo[el] || r.e[k] != m.e[el] || r.elem_bc[k] != m.elem_bc[el];
+        for (size_t c = 0; c < 8; ++c) {
+            const int32_t node = m.nodelist[c * ne + el];
+            bad |= r.nodelist[c * ne + k] != b.node_new[size_t(node)];
+            // a boundary node's corner has the slot of its corner-list entry
+            const int32_t sl = slots[c * ne + k];
+            const int32_t nn_new = b.node_new[size_t(node)];
+            if (nn_new < first_boundary) {
+                bad |= sl != -1;
+            } else {
+                const int32_t j = sl + r.corner_start[size_t(first_boundary)];
+                bad |= sl < 0 || j < r.corner_start[size_t(nn_new)] ||
+                       j >= r.corner_start[size_t(nn_new) + 1] ||
+                       size_t(r.corner_list[size_t(j)]) != c * ne + k;
+            }
+        }
+        for (size_t f = 0; f < 6; ++f) {
+            bad |= r.nbr[f * ne + k] != b.elem_new[size_t(m.nbr[f * ne + el])];
+        }
+    }
+    std::printf("force blocks %s: %s %d blocks, %d interior and %zu boundary nodes\n", name,
+                bad ? "FAILED" : "ok", b.num_blocks, first_boundary,
+                nn - size_t(first_boundary));
+    return bad;
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
@@ -142,6 +237,7 @@ int main(int argc, char **argv)
             lulesh::MeshLists l = lulesh::read_lmesh(argv[i]);
             const lulesh::HostMeshU m = lulesh::build_mesh_u(l);
             std::printf("mesh %s: ok %d %d %d\n", argv[i], m.num_elem, m.num_node, m.origin);
+            bad |= check_force_blocks(m, argv[i]);
             if (!have_first) {
                 first = l;
                 have_first = true;

@@ -108,3 +108,22 @@ def test_write_unstructured_vtk(lulesh_gpu, oracle, tmp_path):
 def test_cube_nodelist(lulesh_gpu, oracle):
     d = lulesh_gpu.Domain(6, ic_mode=1, max_cycles=1)
     assert np.array_equal(d.nodelist(), oracle.cube_mesh(6)["nodelist"])
+
+
+def test_force_blocks_cut(lulesh_gpu, oracle):
+    """How the force kernel cuts the mesh: every node interior to one block
+    or a boundary node, the corners written to device memory at most all of
+    them, and on the 30^3 cube more than half the nodes summed on chip
+    (blocks of 512 elements by coordinate bisection, about 8x8x8)."""
+    d = lulesh_gpu.Domain.from_mesh(oracle.cube_mesh(30))
+    fb = d.force_blocks()
+    print("force blocks 30^3:", fb)
+    assert fb["blocks"] == (30 ** 3 + 511) // 512
+    assert fb["interior_nodes"] + fb["boundary_nodes"] == 31 ** 3
+    assert fb["interior_nodes"] > 0.5 * 31 ** 3
+    assert 0 < fb["boundary_corners"] < 8 * 30 ** 3
+    d.close()
+    d = lulesh_gpu.Domain(6)
+    with pytest.raises(lulesh_gpu.LuleshError):
+        d.force_blocks()
+    d.close()

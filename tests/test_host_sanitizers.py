@@ -65,5 +65,6 @@ def test_host_code_under_sanitizers(tmp_path):
     assert "host code clean" in out.stdout
     assert "sedov15oct.lmesh: ok 875 1126" in out.stdout
     assert "sedov21oct.lmesh: ok 31408 34446" in out.stdout
+    assert "force blocks" in out.stdout and "FAILED" not in out.stdout
     for name in list(broken) + ["missing"]:
         assert f"{name}.lmesh: rejected" in out.stdout, name
