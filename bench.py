@@ -128,9 +128,10 @@ def cpu_baseline(nx: int, cycles: int) -> dict | None:
 
 
 def load_traffic(kernel: str, nx: int) -> float | None:
-    """HBM bytes per launch of `kernel` measured by rocprofv3 --pmc
-    (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), committed under
-    profiles/ by the profiling runs; None when not measured."""
+    """HBM bytes per launch of `kernel` measured by rocprofv3 --pmc (the
+    L2-to-fabric read and write request counts by request size, an
+    accounting a 1 GiB copy reproduces exactly), committed under profiles/
+    by the profiling runs; None when not measured."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
