@@ -127,3 +127,41 @@ def test_force_blocks_cut(lulesh_gpu, oracle):
     with pytest.raises(lulesh_gpu.LuleshError):
         d.force_blocks()
     d.close()
+
+
+def test_cube_lists_against_oracle(lulesh_gpu, oracle):
+    """The 24^3 cube given as lists, cut into 27 element blocks and renumbered
+    on the device, against the oracle's run of the same lists: every field
+    bit for bit after 100 cycles, in the caller's numbering."""
+    mesh = oracle.cube_mesh(24)
+    d = lulesh_gpu.Domain.from_mesh(mesh, max_cycles=100)
+    o = oracle.Oracle.from_mesh(mesh, cbrt_mode=1, threads=16)
+    assert d.force_blocks()["blocks"] == 27
+    assert d.run() == (0, 100)
+    assert o.run(100) == 0 and o.cycle == 100
+    assert d.stats().e_origin == oracle.lib().lo_scalar(o.h, b"e_origin")
+    for f in FIELDS:
+        assert np.array_equal(bits(d.get(f)), bits(o.get(f))), f
+
+
+def test_unstructured_fields_in_callers_numbering(lulesh_gpu, oracle):
+    """Field copies map the device's block numbering back: a node field and
+    an element field set to their own indices read back unchanged, the state
+    survives download and upload, and the nodelist and origin element are
+    the caller's."""
+    mesh = oracle.cube_mesh(12)
+    d = lulesh_gpu.Domain.from_mesh(mesh, max_cycles=5)
+    d.run()
+    nn, ne = 13 ** 3, 12 ** 3
+    d.set("xd", np.arange(nn, dtype=np.float64))
+    d.set("q", np.arange(ne, dtype=np.float64))
+    assert np.array_equal(d.get("xd"), np.arange(nn, dtype=np.float64))
+    assert np.array_equal(d.get("q"), np.arange(ne, dtype=np.float64))
+    st = d.download_state()
+    d.upload_state(st)
+    st2 = d.download_state()
+    for k in ("x", "xd", "e", "p", "q", "v", "ss"):
+        assert np.array_equal(st[k], st2[k]), k
+    assert np.array_equal(d.nodelist(), np.asarray(mesh["nodelist"]).reshape(-1, 8))
+    assert d.origin == 0
+    d.close()
