@@ -128,8 +128,9 @@ void cut_work(lulesh_batch *b, bool all)
                 }
             }
         }
-        // last chunk first: the schedule's costly regions come last, and
-        // long blocks should start before short ones
+        // member by member, last chunk first (launching them in the order the
+        // single domain gives its Q/EOS blocks, the costly ones spread over
+        // the first part, made the ensemble 1.2% slower)
         const int nchunk = (b->f[m].num_elem + kChunk - 1) / kChunk;
         for (int c = nchunk - 1; c >= 0; --c) {
             b->chunks.push_back(make_int2(m, c * kChunk));

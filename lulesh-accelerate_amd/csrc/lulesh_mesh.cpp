@@ -300,20 +300,31 @@ constexpr double kHeavyCost = 1.0, kHeavySpan = 0.75;
 
 }  // namespace
 
-std::vector<int32_t> eos_block_order(const std::vector<uint16_t> &rep)
+std::vector<int64_t> eos_block_passes(const std::vector<uint16_t> &rep)
 {
     const size_t ne = rep.size(), be = size_t(kEosBlockElems);
     const size_t g = (ne + be - 1) / be;
     std::vector<int64_t> passes(g);
-    int64_t sum = 0;
     for (size_t b = 0; b < g; ++b) {
         int64_t extra = 0;
         for (size_t i = b * be; i < std::min(ne, (b + 1) * be); ++i) {
             extra += int64_t(rep[i]) - 1;
         }
         passes[b] = 1 + (extra + int64_t(be) - 1) / int64_t(be);
-        sum += passes[b];
     }
+    return passes;
+}
+
+std::vector<int32_t> eos_block_order(const std::vector<uint16_t> &rep)
+{
+    return spread_costly_blocks(eos_block_passes(rep));
+}
+
+std::vector<int32_t> spread_costly_blocks(const std::vector<int64_t> &passes)
+{
+    const size_t g = passes.size();
+    int64_t sum = 0;
+    for (int64_t p : passes) sum += p;
     const double heavy_min = kHeavyCost * double(sum) / double(std::max<size_t>(g, 1));
     std::vector<int32_t> heavy, rest;
     for (size_t b = 0; b < g; ++b) {

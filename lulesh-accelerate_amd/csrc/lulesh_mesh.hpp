@@ -49,6 +49,12 @@ constexpr int kEosBlockElems = 256;
 // blocks of kEosBlockElems consecutive elements (eos_block_order in
 // lulesh_mesh.cpp): slot t of the launch processes block order[t].
 std::vector<int32_t> eos_block_order(const std::vector<uint16_t> &rep);
+// Its two parts: the EOS passes of each block (one, plus the evaluations
+// beyond each element's first shared over the block), and the launch order
+// of blocks of the given costs, listed in the order whose locality the
+// launch should keep.
+std::vector<int64_t> eos_block_passes(const std::vector<uint16_t> &rep);
+std::vector<int32_t> spread_costly_blocks(const std::vector<int64_t> &passes);
 
 // Builds the mesh, masses, ICs and regions; throws std::invalid_argument.
 HostMesh build_mesh(const MeshConfig &cfg);
