@@ -195,11 +195,20 @@ int queue_cycles(lulesh_batch *b, int ncycles)
         BHIP_TRY(b, hipGetLastError());
         BHIP_TRY(b, launch_force_sweep_batch(b->sweep, gs, b->d_members, b->d_work,
                                              b->prm, st));
-        hipLaunchKernelGGL(k_kinematics_q_batch, dim3(ge), dim3(kChunk), 0, st,
-                           b->d_members, b->d_work, b->prm);
-        BHIP_TRY(b, hipGetLastError());
-        hipLaunchKernelGGL(k_q_eos_dt_batch, dim3(ge), dim3(kChunk), 0, st,
-                           b->d_members, b->d_work, b->prm);
+        if (b->elem_blocks == 0) {
+            // one chunk per block over the full chunk list
+            hipLaunchKernelGGL(k_kinematics_q_chunks, dim3(ge), dim3(kChunk), 0, st,
+                               b->d_members, b->d_work, b->prm);
+            BHIP_TRY(b, hipGetLastError());
+            hipLaunchKernelGGL(k_q_eos_dt_chunks, dim3(ge), dim3(kChunk), 0, st,
+                               b->d_members, b->d_work, b->prm);
+        } else {
+            hipLaunchKernelGGL(k_kinematics_q_batch, dim3(ge), dim3(kChunk), 0, st,
+                               b->d_members, b->d_work, b->prm);
+            BHIP_TRY(b, hipGetLastError());
+            hipLaunchKernelGGL(k_q_eos_dt_batch, dim3(ge), dim3(kChunk), 0, st,
+                               b->d_members, b->d_work, b->prm);
+        }
         BHIP_TRY(b, hipGetLastError());
     }
     hipLaunchKernelGGL(k_dt_finish_batch, dim3(1), dim3(kTimeBlock), 0, st, b->d_members,

@@ -58,12 +58,11 @@ constexpr int kStencil[8][6] = {{1, 2, 3, 4, 5, 7}, {2, 3, 0, 5, 6, 4},
 // (the barriers keep the compiler from moving loads up, so a load issued in
 // the layer that uses it waits for HBM), and the owned node is copied to
 // registers so that the next node plane is stored inside the layer's last
-// fold: four barriers per layer.  The ensemble's tile loop has no registers
-// to spare for these (they spill there), so its kernel stores each plane at
+// fold: four barriers per layer.  Without it the kernel stores each plane at
 // the top of the layer (six barriers) and loads the fields where they are
 // used.  Both give the same bits.
-template <int TX, int TY, bool Fast>
-__device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
+template <int TX, int TY, bool Fast, class Fields>
+__device__ __forceinline__ void sweep_tile(const Fields &f, DevScalars *s,
                                            const PhysParams &prm, int nz, int bx,
                                            int by, int bz)
 {
@@ -90,8 +89,9 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
     const double dt = s_dt;
     const int nx = f.nx;
     const int en = nx + 1;
-    const double *__restrict__ in[6] = {f.x[par ^ 1], f.y[par ^ 1], f.z[par ^ 1],
-                                        f.xd[par ^ 1], f.yd[par ^ 1], f.zd[par ^ 1]};
+    typename Fields::CPtr __restrict__ in[6] = {
+        parity_copy(f.x, par ^ 1),  parity_copy(f.y, par ^ 1),  parity_copy(f.z, par ^ 1),
+        parity_copy(f.xd, par ^ 1), parity_copy(f.yd, par ^ 1), parity_copy(f.zd, par ^ 1)};
 
     const int tx = tid % TX;
     const int ty = tid / TX;
@@ -305,7 +305,8 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
             if constexpr (Fast) {
                 return ev[i];
             } else {
-                const double *const a[6] = {f.p, f.q, f.v, f.volo, f.ss, f.elem_mass};
+                const typename Fields::CPtr a[6] = {f.p, f.q, f.v, f.volo, f.ss,
+                                                    f.elem_mass};
                 return a[i][el];
             }
         };
@@ -473,12 +474,12 @@ __device__ __forceinline__ void sweep_tile(const DevFields &f, DevScalars *s,
             if (fabs(yd) < prm.u_cut) yd = 0.0;
             double zd = own[5] + zdd * dt;
             if (fabs(zd) < prm.u_cut) zd = 0.0;
-            f.xd[par][n] = xd;
-            f.yd[par][n] = yd;
-            f.zd[par][n] = zd;
-            f.x[par][n] = own[0] + xd * dt;
-            f.y[par][n] = own[1] + yd * dt;
-            f.z[par][n] = own[2] + zd * dt;
+            parity_copy(f.xd, par)[n] = xd;
+            parity_copy(f.yd, par)[n] = yd;
+            parity_copy(f.zd, par)[n] = zd;
+            parity_copy(f.x, par)[n] = own[0] + xd * dt;
+            parity_copy(f.y, par)[n] = own[1] + yd * dt;
+            parity_copy(f.z, par)[n] = own[2] + zd * dt;
         }
     }
 }
@@ -499,11 +500,12 @@ __global__ __launch_bounds__(TX *TY, W) void k_force_sweep_batch(
     const BatchMember *__restrict__ members, const BatchWork *__restrict__ work, PhysParams prm, int nz)
 {
     const int n = work->num_tiles;
+    const LULESH_AS_GLOBAL int32_t *tiles = (const LULESH_AS_GLOBAL int32_t *)work->tiles;
     for (int t = int(blockIdx.x); t < n; t += int(gridDim.x)) {
-        const int4 w = work->tiles[t];
         __syncthreads();  // the LDS of the previous tile is no longer read
-        const BatchMember &m = members[w.x];
-        sweep_tile<TX, TY, false>(m.f, m.s, prm, nz, w.y, w.z, w.w);
+        const BatchMember &m = members[tiles[4 * t]];
+        sweep_tile<TX, TY, true>(global_view(m.f), m.s, prm, nz, tiles[4 * t + 1],
+                                  tiles[4 * t + 2], tiles[4 * t + 3]);
     }
 }
 

@@ -79,7 +79,8 @@ __device__ __forceinline__ void apply_constraints(DevScalars *s, int ic_mode)
 }
 
 // element (i, j, k) of a flat element index
-__device__ __forceinline__ void elem_ijk(const DevFields &f, int el, int &i, int &j,
+template <class F>
+__device__ __forceinline__ void elem_ijk(const F &f, int el, int &i, int &j,
                                          int &k)
 {
     const unsigned q1 = fast_div(unsigned(el), f.div_nx);
@@ -180,18 +181,20 @@ namespace {
 
 // the element's eight nodes are given (structured or from a node list);
 // OneTrip: all node loads before any arithmetic (see below)
-template <bool OneTrip>
-__device__ __forceinline__ void kinematics_q_nodes(const DevFields &f, DevScalars *s,
+template <bool OneTrip, class F>
+__device__ __forceinline__ void kinematics_q_nodes(const F &f, DevScalars *s,
                                                    int el, const int (&nodes)[8])
 {
-    const int par = s->cur;
-    const double dt = s->deltatime;
-    const double *__restrict__ px = f.x[par];
-    const double *__restrict__ py = f.y[par];
-    const double *__restrict__ pz = f.z[par];
-    const double *__restrict__ pxd = f.xd[par];
-    const double *__restrict__ pyd = f.yd[par];
-    const double *__restrict__ pzd = f.zd[par];
+    // the loop scalars are read through a global pointer: scalar loads
+    const LULESH_AS_GLOBAL DevScalars *gs = (const LULESH_AS_GLOBAL DevScalars *)s;
+    const int par = wave_uniform(gs->cur);
+    const double dt = wave_uniform(gs->deltatime);
+    typename F::CPtr __restrict__ px = parity_copy(f.x, par);
+    typename F::CPtr __restrict__ py = parity_copy(f.y, par);
+    typename F::CPtr __restrict__ pz = parity_copy(f.z, par);
+    typename F::CPtr __restrict__ pxd = parity_copy(f.xd, par);
+    typename F::CPtr __restrict__ pyd = parity_copy(f.yd, par);
+    typename F::CPtr __restrict__ pzd = parity_copy(f.zd, par);
     double x[8], y[8], z[8], xv[8], yv[8], zv[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
@@ -300,7 +303,8 @@ __device__ __forceinline__ void kinematics_q_nodes(const DevFields &f, DevScalar
     f.delv_eta[el] = ax * dxv + ay * dyv + az * dzv;
 }
 
-__device__ __forceinline__ void kinematics_q(const DevFields &f, DevScalars *s, int el)
+template <class F>
+__device__ __forceinline__ void kinematics_q(const F &f, DevScalars *s, int el)
 {
     int i, j, k;
     elem_ijk(f, el, i, j, k);
@@ -332,16 +336,38 @@ __global__ __launch_bounds__(256) void k_kinematics_q_batch(const BatchMember *_
                                                             PhysParams prm)
 {
     const int n = work->num_chunks;
+    const LULESH_AS_GLOBAL int32_t *chunks = (const LULESH_AS_GLOBAL int32_t *)work->chunks;
     for (int c = int(blockIdx.x); c < n; c += int(gridDim.x)) {
-        const int2 w = work->chunks[c];
-        const BatchMember &m = members[w.x];
-        if (m.s->active == 0 || m.s->error != 0) {
+        const BatchMember &m = members[chunks[2 * c]];
+        if (!member_running(m.s)) {
             continue;
         }
-        const int el = w.y + int(threadIdx.x);
+        const int el = chunks[2 * c + 1] + int(threadIdx.x);
         if (el < m.f.num_elem) {
-            kinematics_q(m.f, m.s, el);
+            kinematics_q(global_view(m.f), m.s, el);
         }
+    }
+}
+
+// The ensemble's element phases with one chunk per block (the default
+// launch: a grid of the full chunk list; blocks beyond the current list
+// return), without the persistent loop.
+__global__ __launch_bounds__(256, 3) void k_kinematics_q_chunks(const BatchMember *__restrict__ members,
+                                                             const BatchWork *__restrict__ work,
+                                                             PhysParams prm)
+{
+    const int c = int(blockIdx.x);
+    if (c >= work->num_chunks) {
+        return;
+    }
+    const LULESH_AS_GLOBAL int32_t *chunks = (const LULESH_AS_GLOBAL int32_t *)work->chunks;
+    const BatchMember &m = members[chunks[2 * c]];
+    if (!member_running(m.s)) {
+        return;
+    }
+    const int el = chunks[2 * c + 1] + int(threadIdx.x);
+    if (el < m.f.num_elem) {
+        kinematics_q(global_view(m.f), m.s, el);
     }
 }
 
@@ -423,10 +449,12 @@ struct QInputs {
 };
 
 // the rest of the inputs once the limiter values are known
-__device__ __forceinline__ void q_inputs_rest(const DevFields &f, int el, double dvxi,
+template <class F>
+__device__ __forceinline__ void q_inputs_rest(const F &f, int el, double dvxi,
                                               double dveta, double dvzeta, QInputs &in);
 
-__device__ __forceinline__ void q_inputs(const DevFields &f, const PhysParams &prm,
+template <class F>
+__device__ __forceinline__ void q_inputs(const F &f, const PhysParams &prm,
                                          int el, int i, int j, int k, QInputs &in)
 {
     const int nx = f.nx;
@@ -483,7 +511,8 @@ __device__ __forceinline__ void q_inputs_u(const DevFields &f, const DevMeshU &m
                          face_value(dvzeta, zp, bc, kBcZetaPSymm, kBcZetaPFree), lm, ms);
 }
 
-__device__ __forceinline__ void q_inputs_rest(const DevFields &f, int el, double dvxi,
+template <class F>
+__device__ __forceinline__ void q_inputs_rest(const F &f, int el, double dvxi,
                                               double dveta, double dvzeta, QInputs &in)
 {
     in.vdov = f.vdov[el];
@@ -558,7 +587,8 @@ __device__ __forceinline__ EosConst eos_const(const PhysParams &prm)
 // qstop tested on the q of the previous cycle (:2368-2381), the new and old
 // relative volumes clamped (:2704-2753; 2.0 clamps vnew in place) and the
 // clamped old volume tested.
-__device__ __forceinline__ EosIn eos_prepare(const DevFields &f, DevScalars *s,
+template <class F>
+__device__ __forceinline__ EosIn eos_prepare(const F &f, DevScalars *s,
                                              const PhysParams &prm, int el,
                                              const QInputs &in, double qlin, double qquad)
 {
@@ -614,7 +644,8 @@ __device__ __forceinline__ EosOut eos_repeat(const EosIn &a, int rep, const EosC
 // (UpdateVolumesForElems: 1.0 uses the unclamped vnew, 2.0 the clamped one,
 // lulesh-2.0.3/lulesh.cc:2439-2452) and the Courant and hydro candidates
 // (:2799-2933).
-__device__ __forceinline__ void eos_finish(const DevFields &f, const PhysParams &prm, int el,
+template <class F>
+__device__ __forceinline__ void eos_finish(const F &f, const PhysParams &prm, int el,
                                            const QInputs &in, double vnewc, const EosOut &o,
                                            unsigned long long &cand_c,
                                            unsigned long long &cand_h)
@@ -648,7 +679,8 @@ __device__ __forceinline__ void eos_finish(const DevFields &f, const PhysParams 
 // Everything after ql / qq for one element: qstop, the volume clamps, the
 // EOS with the region repetitions, the volume update and the element's
 // Courant / hydro candidates.
-__device__ __forceinline__ void eos_dt_elem(const DevFields &f, DevScalars *s,
+template <class F>
+__device__ __forceinline__ void eos_dt_elem(const F &f, DevScalars *s,
                                             const PhysParams &prm, int el, int rep,
                                             const QInputs &in, double qlin, double qquad,
                                             unsigned long long &cand_c,
@@ -685,7 +717,8 @@ __device__ __forceinline__ void fold_minima(DevScalars *s, unsigned long long ca
 
 // one block of 256 elements in schedule order; every thread of the block
 // calls it (the minima are reduced over the block)
-__device__ __forceinline__ void q_eos_dt_block(const DevFields &f, DevScalars *s,
+template <class F>
+__device__ __forceinline__ void q_eos_dt_block(const F &f, DevScalars *s,
                                                const PhysParams &prm, int slot,
                                                unsigned long long *lds_c,
                                                unsigned long long *lds_h)
@@ -803,7 +836,8 @@ namespace {
 
 // One block of 256 elements in mesh order, el the thread's element; every
 // thread of the block calls it (the minima are reduced over the block).
-__device__ __forceinline__ void q_eos_dt_nat_block(const DevFields &f, DevScalars *s,
+template <class F>
+__device__ __forceinline__ void q_eos_dt_nat_block(const F &f, DevScalars *s,
                                                    const PhysParams &prm, int el,
                                                    unsigned long long *lds_c,
                                                    unsigned long long *lds_h, RepLds &L)
@@ -878,25 +912,46 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_batch(const BatchMember *__res
     __shared__ int skip;
     __shared__ RepLds rep_lds;
     const int n = work->num_chunks;
+    const LULESH_AS_GLOBAL int32_t *chunks = (const LULESH_AS_GLOBAL int32_t *)work->chunks;
     for (int c = int(blockIdx.x); c < n; c += int(gridDim.x)) {
-        const int2 w = work->chunks[c];
-        // uniform_member keeps the member's fields in scalar registers here
-        // (measured: the ensemble 2% faster; in the sweep and kinematics
-        // kernels the same change was slower, so they load it directly)
-        const BatchMember m = uniform_member(members[w.x]);
+        const BatchMember &m = members[chunks[2 * c]];
         __syncthreads();  // the previous chunk's reads of the LDS are done
         if (threadIdx.x == 0) {
-            skip = (m.s->active == 0 || m.s->error != 0) ? 1 : 0;
+            skip = member_running(m.s) ? 0 : 1;
         }
         __syncthreads();
         if (skip) {
             continue;
         }
         __builtin_amdgcn_s_setprio(1);  // the element's loads first
-        q_eos_dt_nat_block(m.f, m.s, prm,
-                           __builtin_amdgcn_readfirstlane(w.y) + int(threadIdx.x), lds_c,
-                           lds_h, rep_lds);
+        q_eos_dt_nat_block(global_view(m.f), m.s, prm, chunks[2 * c + 1] + int(threadIdx.x),
+                           lds_c, lds_h, rep_lds);
     }
+}
+
+__global__ __launch_bounds__(256) void k_q_eos_dt_chunks(const BatchMember *__restrict__ members,
+                                                         const BatchWork *__restrict__ work,
+                                                         PhysParams prm)
+{
+    __shared__ unsigned long long lds_c[4], lds_h[4];
+    __shared__ int skip;
+    __shared__ RepLds rep_lds;
+    const int c = int(blockIdx.x);
+    if (c >= work->num_chunks) {
+        return;
+    }
+    const LULESH_AS_GLOBAL int32_t *chunks = (const LULESH_AS_GLOBAL int32_t *)work->chunks;
+    const BatchMember &m = members[chunks[2 * c]];
+    if (threadIdx.x == 0) {
+        skip = member_running(m.s) ? 0 : 1;
+    }
+    __syncthreads();
+    if (skip) {
+        return;
+    }
+    __builtin_amdgcn_s_setprio(1);  // the element's loads first
+    q_eos_dt_nat_block(global_view(m.f), m.s, prm, chunks[2 * c + 1] + int(threadIdx.x),
+                       lds_c, lds_h, rep_lds);
 }
 
 // ---------------------------------------------------------------------------

@@ -112,6 +112,7 @@ __device__ __forceinline__ int eos_chunk(int b, int g, int bands)
 }
 
 struct DevFields {
+    using CPtr = const double *;  // a read-only field (see DevFieldsG)
     int nx;
     int num_elem, num_node;
     FastDiv div_nx;
@@ -130,6 +131,116 @@ struct DevFields {
     // dispatch order of the mesh-order Q/EOS blocks (eos_block_order)
     const int32_t *eos_blocks;
 };
+
+// The same fields as pointers into the global address space, for device code
+// that finds a domain's field table in memory (the ensemble's members).  A
+// pointer loaded from memory is generic to the compiler, which then reaches
+// every field through flat instructions, each of which counts against both
+// memory counters, so that every wait for a load waits for all of them; a
+// global pointer gives global loads and stores, as the single domain's kernel
+// arguments do.  The element functions take either table (template class F).
+#define LULESH_AS_GLOBAL __attribute__((address_space(1)))
+struct DevFieldsG {
+    using CPtr = const LULESH_AS_GLOBAL double *;
+    int nx;
+    int num_elem, num_node;
+    FastDiv div_nx;
+    LULESH_AS_GLOBAL double *x[2], *y[2], *z[2], *xd[2], *yd[2], *zd[2];
+    LULESH_AS_GLOBAL double *nodal_mass;
+    LULESH_AS_GLOBAL double *e, *p, *q, *ql, *qq, *v, *volo, *elem_mass, *ss;
+    LULESH_AS_GLOBAL double *vnew, *delv, *arealg, *vdov;
+    LULESH_AS_GLOBAL double *delv_xi, *delv_eta, *delv_zeta, *delx_xi, *delx_eta, *delx_zeta;
+    const LULESH_AS_GLOBAL int32_t *eos_order;
+    const LULESH_AS_GLOBAL uint16_t *eos_rep;
+    const LULESH_AS_GLOBAL uint16_t *rep;
+    const LULESH_AS_GLOBAL int32_t *eos_blocks;
+};
+
+// the global view of a field table (every pointer of it addresses device
+// memory allocated by the library)
+__device__ __forceinline__ DevFieldsG global_view(const DevFields &f)
+{
+    DevFieldsG g;
+    g.nx = f.nx;
+    g.num_elem = f.num_elem;
+    g.num_node = f.num_node;
+    g.div_nx = f.div_nx;
+    for (int c = 0; c < 2; ++c) {
+        g.x[c] = (LULESH_AS_GLOBAL double *)f.x[c];
+        g.y[c] = (LULESH_AS_GLOBAL double *)f.y[c];
+        g.z[c] = (LULESH_AS_GLOBAL double *)f.z[c];
+        g.xd[c] = (LULESH_AS_GLOBAL double *)f.xd[c];
+        g.yd[c] = (LULESH_AS_GLOBAL double *)f.yd[c];
+        g.zd[c] = (LULESH_AS_GLOBAL double *)f.zd[c];
+    }
+    g.nodal_mass = (LULESH_AS_GLOBAL double *)f.nodal_mass;
+    g.e = (LULESH_AS_GLOBAL double *)f.e;
+    g.p = (LULESH_AS_GLOBAL double *)f.p;
+    g.q = (LULESH_AS_GLOBAL double *)f.q;
+    g.ql = (LULESH_AS_GLOBAL double *)f.ql;
+    g.qq = (LULESH_AS_GLOBAL double *)f.qq;
+    g.v = (LULESH_AS_GLOBAL double *)f.v;
+    g.volo = (LULESH_AS_GLOBAL double *)f.volo;
+    g.elem_mass = (LULESH_AS_GLOBAL double *)f.elem_mass;
+    g.ss = (LULESH_AS_GLOBAL double *)f.ss;
+    g.vnew = (LULESH_AS_GLOBAL double *)f.vnew;
+    g.delv = (LULESH_AS_GLOBAL double *)f.delv;
+    g.arealg = (LULESH_AS_GLOBAL double *)f.arealg;
+    g.vdov = (LULESH_AS_GLOBAL double *)f.vdov;
+    g.delv_xi = (LULESH_AS_GLOBAL double *)f.delv_xi;
+    g.delv_eta = (LULESH_AS_GLOBAL double *)f.delv_eta;
+    g.delv_zeta = (LULESH_AS_GLOBAL double *)f.delv_zeta;
+    g.delx_xi = (LULESH_AS_GLOBAL double *)f.delx_xi;
+    g.delx_eta = (LULESH_AS_GLOBAL double *)f.delx_eta;
+    g.delx_zeta = (LULESH_AS_GLOBAL double *)f.delx_zeta;
+    g.eos_order = (const LULESH_AS_GLOBAL int32_t *)f.eos_order;
+    g.eos_rep = (const LULESH_AS_GLOBAL uint16_t *)f.eos_rep;
+    g.rep = (const LULESH_AS_GLOBAL uint16_t *)f.rep;
+    g.eos_blocks = (const LULESH_AS_GLOBAL int32_t *)f.eos_blocks;
+    return g;
+}
+
+// A value every lane of the wave holds, moved to a scalar register.  A
+// field table found in memory reaches the loop scalars through a generic
+// pointer, whose loads land in vector registers; the parity then selects
+// every field pointer per lane.  Where the value is already scalar (the
+// single domain's kernel arguments) this compiles to nothing.
+__device__ __forceinline__ int wave_uniform(int v)
+{
+    return __builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ double wave_uniform(double v)
+{
+    const unsigned long long b = __double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readfirstlane(unsigned(b));
+    const unsigned hi = __builtin_amdgcn_readfirstlane(unsigned(b >> 32));
+    return __longlong_as_double((long long)((unsigned long long)hi << 32 | lo));
+}
+
+// One of a node field's two parity copies.  The single domain's table is a
+// kernel argument and is indexed where it lies; the global view is a copy
+// in registers, which a run-time index would move to scratch memory, so its
+// pointer is selected instead.
+__device__ __forceinline__ double *parity_copy(double *const (&t)[2], int par)
+{
+    return t[par];
+}
+__device__ __forceinline__ LULESH_AS_GLOBAL double *
+parity_copy(LULESH_AS_GLOBAL double *const (&t)[2], int par)
+{
+    return par != 0 ? t[1] : t[0];
+}
+
+// Whether an ensemble member takes part in the cycle: active and without
+// error.  Both flags are read before either is tested (a short-circuit test
+// waits for the first load before it issues the second).
+__device__ __forceinline__ bool member_running(const DevScalars *s)
+{
+    const LULESH_AS_GLOBAL DevScalars *g = (const LULESH_AS_GLOBAL DevScalars *)s;
+    const int active = g->active;
+    const int error = g->error;
+    return (active != 0) & (error == 0);
+}
 
 // The mixed-precision domain: the node arrays, the energy, the volumes and
 // the masses stay float64; the element-local quantities of the stress,
@@ -165,26 +276,6 @@ struct BatchMember {
     DevScalars *s;
 };
 
-// A member read from the work lists is the same for every lane of a wave,
-// but the compiler cannot prove it from a loaded index, so it would keep
-// every field pointer in vector registers.  Passing each 32-bit word
-// through readfirstlane marks the copy wave-uniform: the fields a kernel
-// uses then live in scalar registers.  The values are unchanged.
-__device__ __forceinline__ BatchMember uniform_member(const BatchMember &m)
-{
-    static_assert(sizeof(BatchMember) % 4 == 0, "BatchMember is a whole number of words");
-    constexpr int kWords = int(sizeof(BatchMember) / 4);
-    int w[kWords];
-    __builtin_memcpy(w, &m, sizeof(BatchMember));
-#pragma unroll
-    for (int i = 0; i < kWords; ++i) {
-        w[i] = __builtin_amdgcn_readfirstlane(w[i]);
-    }
-    BatchMember u;
-    __builtin_memcpy(&u, w, sizeof(BatchMember));
-    return u;
-}
-
 // Work lists of the persistent ensemble kernels, in device memory.  The
 // host rewrites them (between cycles) when members finish, so the kernels
 // read the counts from here rather than from their launch arguments.
@@ -213,6 +304,10 @@ __global__ void k_kinematics_q_batch(const BatchMember *members,
                                      const BatchWork *work, PhysParams prm);
 __global__ void k_q_eos_dt_batch(const BatchMember *members, const BatchWork *work,
                                  PhysParams prm);
+__global__ void k_kinematics_q_chunks(const BatchMember *members, const BatchWork *work,
+                                      PhysParams prm);
+__global__ void k_q_eos_dt_chunks(const BatchMember *members, const BatchWork *work,
+                                  PhysParams prm);
 __global__ void k_dt_finish_batch(const BatchMember *members, BatchWork *work,
                                   int ic_mode);
 
