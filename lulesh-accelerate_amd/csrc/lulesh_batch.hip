@@ -68,9 +68,12 @@ struct lulesh_batch {
     BatchWork *d_work = nullptr;
     int4 *d_tiles = nullptr;
     int2 *d_chunks = nullptr;
+    int2 *d_eos_chunks = nullptr;
     int *h_pending = nullptr;  // pinned
     std::vector<int4> tiles;
     std::vector<int2> chunks;
+    std::vector<int2> eos_chunks;
+    std::vector<std::vector<int64_t>> eos_passes;  // EOS passes of each member's chunks
     std::vector<char> dropped;  // members no longer in the work lists
     int full_tiles = 0, full_chunks = 0;
 
@@ -107,6 +110,33 @@ int bfail(lulesh_batch *b, int code, const std::string &msg)
         }                                                                    \
     } while (0)
 
+// The Q/EOS chunks of the members still in the lists, in dispatch order.  A
+// chunk is one of its member's Q/EOS blocks (kChunk = kEosBlockElems), and
+// the blocks' costs are as uneven over the ensemble as over one domain, so
+// the whole list is ordered as the single domain orders its blocks
+// (spread_costly_blocks): the costly chunks of all members spread over each
+// XCD's first slots, the rest member by member in mesh order.
+void order_eos_chunks(lulesh_batch *b, bool all)
+{
+    std::vector<int2> base;
+    std::vector<int64_t> passes;
+    for (int m = 0; m < b->n; ++m) {
+        if (!all && b->dropped[m]) {
+            continue;
+        }
+        const std::vector<int64_t> &p = b->eos_passes[size_t(m)];
+        for (size_t c = 0; c < p.size(); ++c) {
+            base.push_back(make_int2(m, int(c) * kChunk));
+            passes.push_back(p[c]);
+        }
+    }
+    const std::vector<int32_t> order = spread_costly_blocks(passes);
+    b->eos_chunks.resize(base.size());
+    for (size_t t = 0; t < order.size(); ++t) {
+        b->eos_chunks[t] = base[size_t(order[t])];
+    }
+}
+
 // tiles of the sweep (for the current shape) and element chunks of the
 // members still in the lists
 void cut_work(lulesh_batch *b, bool all)
@@ -128,14 +158,13 @@ void cut_work(lulesh_batch *b, bool all)
                 }
             }
         }
-        // member by member, last chunk first (launching them in the order the
-        // single domain gives its Q/EOS blocks, the costly ones spread over
-        // the first part, made the ensemble 1.2% slower)
+        // member by member, last chunk first
         const int nchunk = (b->f[m].num_elem + kChunk - 1) / kChunk;
         for (int c = nchunk - 1; c >= 0; --c) {
             b->chunks.push_back(make_int2(m, c * kChunk));
         }
     }
+    order_eos_chunks(b, all);
 }
 
 // uploads the work lists (and their counts) for the members not dropped;
@@ -150,6 +179,7 @@ int upload_work(lulesh_batch *b)
     w.num_pending = 0;
     w.tiles = b->d_tiles;
     w.chunks = b->d_chunks;
+    w.eos_chunks = b->d_eos_chunks;
     if (!b->tiles.empty()) {
         BHIP_TRY(b, hipMemcpyAsync(b->d_tiles, b->tiles.data(),
                                    b->tiles.size() * sizeof(int4),
@@ -158,6 +188,9 @@ int upload_work(lulesh_batch *b)
     if (!b->chunks.empty()) {
         BHIP_TRY(b, hipMemcpyAsync(b->d_chunks, b->chunks.data(),
                                    b->chunks.size() * sizeof(int2),
+                                   hipMemcpyHostToDevice, b->stream));
+        BHIP_TRY(b, hipMemcpyAsync(b->d_eos_chunks, b->eos_chunks.data(),
+                                   b->eos_chunks.size() * sizeof(int2),
                                    hipMemcpyHostToDevice, b->stream));
     }
     BHIP_TRY(b, hipMemcpyAsync(b->d_work, &w, sizeof(w), hipMemcpyHostToDevice,
@@ -354,12 +387,14 @@ int batch_build(lulesh_batch *b, const lulesh_config *cfg, const double *efactor
     std::vector<BatchMember> members(size_t(b->n));
     std::vector<DevScalars> scalars(size_t(b->n));
     b->f.resize(size_t(b->n));
+    b->eos_passes.resize(size_t(b->n));
     for (int m = 0; m < b->n; ++m) {
         DevScalars *unused = nullptr;  // the members' scalars live contiguously
         carve_domain(static_cast<char *>(b->arena) + offset[m], meshes[m], b->f[m],
                      &unused);
         BHIP_TRY(b, upload_mesh(meshes[m], b->f[m], b->stream));
         members[m].f = b->f[m];
+        b->eos_passes[size_t(m)] = eos_block_passes(meshes[m].elem_rep);
         members[m].s = b->d_scalars + m;
         scalars[m] = initial_scalars(meshes[m], cfg[m].stoptime, cfg[m].max_cycles);
     }
@@ -378,6 +413,7 @@ int batch_build(lulesh_batch *b, const lulesh_config *cfg, const double *efactor
     size_grids(b);
     BHIP_TRY(b, hipMalloc(&b->d_tiles, size_t(std::max(b->full_tiles, 1)) * sizeof(int4)));
     BHIP_TRY(b, hipMalloc(&b->d_chunks, size_t(std::max(b->full_chunks, 1)) * sizeof(int2)));
+    BHIP_TRY(b, hipMalloc(&b->d_eos_chunks, size_t(std::max(b->full_chunks, 1)) * sizeof(int2)));
     return upload_work(b);
 }
 
@@ -472,6 +508,7 @@ void lulesh_batch_destroy(lulesh_batch *b)
     if (b->d_work) (void)hipFree(b->d_work);
     if (b->d_tiles) (void)hipFree(b->d_tiles);
     if (b->d_chunks) (void)hipFree(b->d_chunks);
+    if (b->d_eos_chunks) (void)hipFree(b->d_eos_chunks);
     if (b->h_pending) (void)hipHostFree(b->h_pending);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;

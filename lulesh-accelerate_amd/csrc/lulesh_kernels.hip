@@ -912,7 +912,7 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_batch(const BatchMember *__res
     __shared__ int skip;
     __shared__ RepLds rep_lds;
     const int n = work->num_chunks;
-    const LULESH_AS_GLOBAL int32_t *chunks = (const LULESH_AS_GLOBAL int32_t *)work->chunks;
+    const LULESH_AS_GLOBAL int32_t *chunks = (const LULESH_AS_GLOBAL int32_t *)work->eos_chunks;
     for (int c = int(blockIdx.x); c < n; c += int(gridDim.x)) {
         const BatchMember &m = members[chunks[2 * c]];
         __syncthreads();  // the previous chunk's reads of the LDS are done
@@ -940,7 +940,7 @@ __global__ __launch_bounds__(256) void k_q_eos_dt_chunks(const BatchMember *__re
     if (c >= work->num_chunks) {
         return;
     }
-    const LULESH_AS_GLOBAL int32_t *chunks = (const LULESH_AS_GLOBAL int32_t *)work->chunks;
+    const LULESH_AS_GLOBAL int32_t *chunks = (const LULESH_AS_GLOBAL int32_t *)work->eos_chunks;
     const BatchMember &m = members[chunks[2 * c]];
     if (threadIdx.x == 0) {
         skip = member_running(m.s) ? 0 : 1;

@@ -286,7 +286,8 @@ struct BatchWork {
     int num_chunks;   // element chunks (member, first element)
     int num_pending;  // members with cycles left, set by k_dt_finish_batch
     int4 *tiles;
-    int2 *chunks;
+    int2 *chunks;      // in mesh order, for the kinematics
+    int2 *eos_chunks;  // the same chunks in the Q/EOS dispatch order
 };
 
 __global__ void k_time_increment(DevScalars *s, int ic_mode);
