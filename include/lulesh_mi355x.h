@@ -254,8 +254,10 @@ double lulesh_last_step_ms(lulesh_domain *d);
 const char *lulesh_last_error(const lulesh_domain *d);
 
 /* The library's cube root (lulesh_cbrt.h) evaluated on the GPU (op 0), IEEE
- * sqrt (1) and division (2), and the EOS's square root for inputs >= 2^-767
- * (3), for n inputs: a math-parity probe for the tests. */
+ * sqrt (1) and division (2), the EOS's square root for inputs >= 2^-767
+ * (3), and the quotient of the fused multiply-add build (4: the hardware
+ * reciprocal refined by two Newton steps, times a), for n inputs: a
+ * math-parity probe for the tests. */
 int lulesh_device_math(int op, int n, const double *a, const double *b,
                        double *out);
 

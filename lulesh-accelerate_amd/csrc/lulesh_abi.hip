@@ -1235,7 +1235,8 @@ const char *lulesh_last_error(const lulesh_domain *d)
 int lulesh_device_math(int op, int n, const double *a, const double *b,
                        double *out)
 {
-    if (n < 0 || !a || !out || (op == 2 && !b) || op < 0 || op > 3) {
+    const bool two = op == 2 || op == 4;
+    if (n < 0 || !a || !out || (two && !b) || op < 0 || op > 4) {
         return LULESH_BAD_ARG;
     }
     if (n == 0) {
@@ -1245,17 +1246,23 @@ int lulesh_device_math(int op, int n, const double *a, const double *b,
     const size_t bytes = size_t(n) * sizeof(double);
     int rc = LULESH_OK;
     if (hipMalloc(&da, bytes) != hipSuccess || hipMalloc(&dout, bytes) != hipSuccess ||
-        (op == 2 && hipMalloc(&db, bytes) != hipSuccess)) {
+        (two && hipMalloc(&db, bytes) != hipSuccess)) {
         rc = LULESH_HIP_ERROR;
     }
     if (rc == LULESH_OK && hipMemcpy(da, a, bytes, hipMemcpyHostToDevice) != hipSuccess)
         rc = LULESH_HIP_ERROR;
-    if (rc == LULESH_OK && op == 2 &&
+    if (rc == LULESH_OK && two &&
         hipMemcpy(db, b, bytes, hipMemcpyHostToDevice) != hipSuccess)
         rc = LULESH_HIP_ERROR;
     if (rc == LULESH_OK) {
-        hipLaunchKernelGGL(k_device_math, dim3((n + 255) / 256), dim3(256), 0, 0,
-                           op, n, da, db, dout);
+        if (op == 4) {
+            // the fused build's quotient
+            hipLaunchKernelGGL(fused::k_device_math, dim3((n + 255) / 256), dim3(256), 0, 0,
+                               2, n, da, db, dout);
+        } else {
+            hipLaunchKernelGGL(k_device_math, dim3((n + 255) / 256), dim3(256), 0, 0,
+                               op, n, da, db, dout);
+        }
         if (hipGetLastError() != hipSuccess ||
             hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost) != hipSuccess)
             rc = LULESH_HIP_ERROR;

@@ -44,6 +44,33 @@ namespace lulesh {
 // reference's double constant, so the rounding is unchanged), the
 // mixed-precision step with float on element-local coordinates.
 
+// 1 / b and a / b in the element arithmetic.  The bit-identical build
+// divides, correctly rounded, as the reference does.  The fused build, held
+// to the reference's tolerances rather than to its bits, multiplies by the
+// hardware reciprocal estimate refined by two Newton steps (b finite, normal
+// and nonzero): five instructions where the divide sequence takes ten.  In
+// float (the mixed-precision step) these are plain divides.
+__device__ __forceinline__ double elem_recip(double b)
+{
+#ifdef LULESH_FUSED
+    double r = __builtin_amdgcn_rcp(b);
+    r = __builtin_fma(r, __builtin_fma(-b, r, 1.0), r);
+    r = __builtin_fma(r, __builtin_fma(-b, r, 1.0), r);
+    return r;
+#else
+    return 1.0 / b;
+#endif
+}
+__device__ __forceinline__ double elem_quot(double a, double b)
+{
+#ifdef LULESH_FUSED
+    return a * elem_recip(b);
+#else
+    return a / b;
+#endif
+}
+__device__ __forceinline__ float elem_quot(float a, float b) { return a / b; }
+
 // fast square root of the arithmetic type: IEEE sqrt in double, the
 // hardware v_sqrt_f32 (1 ulp) in float
 __device__ __forceinline__ double elem_sqrt(double v) { return sqrt(v); }
@@ -262,7 +289,7 @@ __device__ __forceinline__ T characteristic_length(const T *x, const T *y, const
     cl = ref_max(area_face(x, y, z, 1, 2, 6, 5), cl);
     cl = ref_max(area_face(x, y, z, 2, 3, 7, 6), cl);
     cl = ref_max(area_face(x, y, z, 3, 0, 4, 7), cl);
-    return T(4.0) * volume / elem_sqrt(cl);
+    return elem_quot(T(4.0) * volume, elem_sqrt(cl));
 }
 
 // Equation of state of one element: CalcEnergyForElems with its three
@@ -341,9 +368,9 @@ __device__ __forceinline__ EosOut eos_element(double vnewc, double e_old,
 {
     const double c1s = 2.0 / 3.0;
     const double work = 0.;
-    double compression = 1. / vnewc - 1.;
+    double compression = elem_recip(vnewc) - 1.;
     const double vchalf = vnewc - delvc * .5;
-    double comp_half_step = 1. / vchalf - 1.;
+    double comp_half_step = elem_recip(vchalf) - 1.;
     if (c.eosvmin != 0.) {
         if (vnewc <= c.eosvmin) comp_half_step = compression;
     }
@@ -361,7 +388,7 @@ __device__ __forceinline__ EosOut eos_element(double vnewc, double e_old,
     double pbvc = c1s;
     const double p_half_step = eos_pressure(bvc, e_new, vnewc, c);
 
-    const double vhalf = 1. / (1. + comp_half_step);
+    const double vhalf = elem_recip(1. + comp_half_step);
     double q_new;
     if (delvc > 0.) {
         q_new = 0.;

@@ -376,11 +376,11 @@ __device__ __forceinline__ void sweep_tile(const Fields &f, DevScalars *s,
             if (compute) {
                 const double vv = field(2);
                 const double determ = field(3) * vv;
-                const double volinv = 1.0 / determ;
+                const double volinv = elem_recip(determ);
                 // the coefficient before hourgam, which holds 32 values live
                 const double volume13 = lulesh_cbrt(determ);
                 const double coefficient =
-                    -prm.hgcoef * 0.01 * field(4) * field(5) / volume13;
+                    elem_quot(-prm.hgcoef * 0.01 * field(4) * field(5), volume13);
                 double hourgam[8][4];
                 {
                     double x[8], y[8], z[8];
@@ -462,9 +462,9 @@ __device__ __forceinline__ void sweep_tile(const Fields &f, DevScalars *s,
                     own[fld] = plane[lo][fld][base];
                 }
             }
-            double xdd = fx / mass;
-            double ydd = fy / mass;
-            double zdd = fz / mass;
+            double xdd = elem_quot(fx, mass);
+            double ydd = elem_quot(fy, mass);
+            double zdd = elem_quot(fz, mass);
             if (ie == 0) xdd = 0.0;
             if (je == 0) ydd = 0.0;
             if (k == 0) zdd = 0.0;
@@ -614,7 +614,7 @@ __device__ __forceinline__ void hourglass_corners_u(const ElemU &E, const PhysPa
                                                     double fh[3][8])
 {
     const double determ = E.volo * E.v;
-    const double volinv = 1.0 / determ;
+    const double volinv = elem_recip(determ);
     double hourgam[8][4];
     {
         double hm[4][3];
@@ -646,7 +646,7 @@ __device__ __forceinline__ void hourglass_corners_u(const ElemU &E, const PhysPa
         }
     }
     const double volume13 = lulesh_cbrt(determ);
-    const double coefficient = -prm.hgcoef * 0.01 * E.ss * E.emass / volume13;
+    const double coefficient = elem_quot(-prm.hgcoef * 0.01 * E.ss * E.emass, volume13);
 #pragma unroll
     for (int t = 0; t < 3; ++t) {
         const double *u = E.vel[t];
@@ -684,9 +684,9 @@ __device__ __forceinline__ void move_node_u(const DevFields &f, int par, double 
                                             const PhysParams &prm, int n, const NodeU &nd,
                                             double fx, double fy, double fz)
 {
-    double xdd = fx / nd.mass;
-    double ydd = fy / nd.mass;
-    double zdd = fz / nd.mass;
+    double xdd = elem_quot(fx, nd.mass);
+    double ydd = elem_quot(fy, nd.mass);
+    double zdd = elem_quot(fz, nd.mass);
     if (nd.sym & 1) xdd = 0.0;
     if (nd.sym & 2) ydd = 0.0;
     if (nd.sym & 4) zdd = 0.0;

@@ -223,7 +223,7 @@ __device__ __forceinline__ void kinematics_q_nodes(const F &f, DevScalars *s,
 
     // kinematics
     const double volume = elem_volume(x, y, z);
-    const double vnew = volume / volo;
+    const double vnew = elem_quot(volume, volo);
     f.vnew[el] = vnew;
     f.delv[el] = vnew - v_old;
     f.arealg[el] = characteristic_length(x, y, z, volume);
@@ -237,7 +237,7 @@ __device__ __forceinline__ void kinematics_q_nodes(const F &f, DevScalars *s,
             zh[c] = z[c] - dt2 * zv[c];
         }
         const double det_j = shape_derivatives(xh, yh, zh, b);
-        const double inv_det_j = 1.0 / det_j;
+        const double inv_det_j = elem_recip(det_j);
         const double dxx = inv_det_j * (b[0][0] * (xv[0] - xv[6]) + b[0][1] * (xv[1] - xv[7]) +
                                         b[0][2] * (xv[2] - xv[4]) + b[0][3] * (xv[3] - xv[5]));
         const double dyy = inv_det_j * (b[1][0] * (yv[0] - yv[6]) + b[1][1] * (yv[1] - yv[7]) +
@@ -254,7 +254,7 @@ __device__ __forceinline__ void kinematics_q_nodes(const F &f, DevScalars *s,
     // ptiny, so its argument is >= 1e-36 and sqrt_above_tiny is sqrt there
     const double ptiny = 1.e-36;
     const double vol = volo * vnew;
-    const double norm = 1.0 / (vol + ptiny);
+    const double norm = elem_recip(vol + ptiny);
     const double dxj = -0.25 * ((x[0] + x[1] + x[5] + x[4]) - (x[3] + x[2] + x[6] + x[7]));
     const double dyj = -0.25 * ((y[0] + y[1] + y[5] + y[4]) - (y[3] + y[2] + y[6] + y[7]));
     const double dzj = -0.25 * ((z[0] + z[1] + z[5] + z[4]) - (z[3] + z[2] + z[6] + z[7]));
@@ -269,7 +269,7 @@ __device__ __forceinline__ void kinematics_q_nodes(const F &f, DevScalars *s,
     ax = dyi * dzj - dzi * dyj;
     ay = dzi * dxj - dxi * dzj;
     az = dxi * dyj - dyi * dxj;
-    f.delx_zeta[el] = vol / sqrt_above_tiny(ax * ax + ay * ay + az * az + ptiny);
+    f.delx_zeta[el] = elem_quot(vol, sqrt_above_tiny(ax * ax + ay * ay + az * az + ptiny));
     ax *= norm;
     ay *= norm;
     az *= norm;
@@ -281,7 +281,7 @@ __device__ __forceinline__ void kinematics_q_nodes(const F &f, DevScalars *s,
     ax = dyj * dzk - dzj * dyk;
     ay = dzj * dxk - dxj * dzk;
     az = dxj * dyk - dyj * dxk;
-    f.delx_xi[el] = vol / sqrt_above_tiny(ax * ax + ay * ay + az * az + ptiny);
+    f.delx_xi[el] = elem_quot(vol, sqrt_above_tiny(ax * ax + ay * ay + az * az + ptiny));
     ax *= norm;
     ay *= norm;
     az *= norm;
@@ -293,7 +293,7 @@ __device__ __forceinline__ void kinematics_q_nodes(const F &f, DevScalars *s,
     ax = dyk * dzi - dzk * dyi;
     ay = dzk * dxi - dxk * dzi;
     az = dxk * dyi - dyk * dxi;
-    f.delx_eta[el] = vol / sqrt_above_tiny(ax * ax + ay * ay + az * az + ptiny);
+    f.delx_eta[el] = elem_quot(vol, sqrt_above_tiny(ax * ax + ay * ay + az * az + ptiny));
     ax *= norm;
     ay *= norm;
     az *= norm;
@@ -379,7 +379,7 @@ __device__ __forceinline__ double mono_phi(double self, double delvm,
                                            double max_slope)
 {
     const double ptiny = 1.e-36;
-    const double norm = 1. / (self + ptiny);
+    const double norm = elem_recip(self + ptiny);
     double phi;
     delvm = delvm * norm;
     delvp = delvp * norm;
@@ -538,7 +538,7 @@ __device__ __forceinline__ void q_inputs_rest(const F &f, int el, double dvxi,
         in.du[0] = dxi;
         in.du[1] = deta;
         in.du[2] = dzeta;
-        in.rho = f.elem_mass[el] / (in.volo * in.vnew);
+        in.rho = elem_quot(f.elem_mass[el], in.volo * in.vnew);
     }
 }
 
@@ -669,9 +669,9 @@ __device__ __forceinline__ void eos_finish(const F &f, const PhysParams &prm, in
             dtf = dtf + qqc2 * arealg * arealg * vdov * vdov;
         }
         dtf = sqrt(dtf);
-        dtf = arealg / dtf;
+        dtf = elem_quot(arealg, dtf);
         cand_c = as_bits(dtf);
-        const double dtdvov = prm.dvovmax / (fabs(vdov) + 1.e-20);
+        const double dtdvov = elem_quot(prm.dvovmax, fabs(vdov) + 1.e-20);
         cand_h = as_bits(dtdvov);
     }
 }
@@ -1226,8 +1226,10 @@ __global__ void k_dt_finish_batch(const BatchMember *members, BatchWork *work,
     }
 }
 
-// device math probe: op 0 = lulesh_cbrt(a), 1 = sqrt(a), 2 = a / b,
-// 3 = sqrt_above_tiny(a) (the EOS's square root)
+// device math probe: op 0 = lulesh_cbrt(a), 1 = sqrt(a), 2 = the element
+// arithmetic's quotient a / b (elem_quot: a divide here, the refined
+// reciprocal in the fused build), 3 = sqrt_above_tiny(a) (the EOS's square
+// root)
 __global__ void k_device_math(int op, int n, const double *a, const double *b,
                               double *out)
 {
@@ -1242,7 +1244,7 @@ __global__ void k_device_math(int op, int n, const double *a, const double *b,
     } else if (op == 3) {
         out[t] = sqrt_above_tiny(a[t]);
     } else {
-        out[t] = a[t] / b[t];
+        out[t] = elem_quot(a[t], b[t]);
     }
 }
 

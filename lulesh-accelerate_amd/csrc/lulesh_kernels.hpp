@@ -389,6 +389,8 @@ __global__ void k_q_eos_dt(DevFields f, DevScalars *s, PhysParams prm, int bands
 __global__ void k_q_eos_dt_nat(DevFields f, DevScalars *s, PhysParams prm);
 __global__ void k_q_eos_dt_learned(DevFields f, DevScalars *s, PhysParams prm,
                                    const MlpFrags *net, float z_max, int form);
+__global__ void k_device_math(int op, int n, const double *a, const double *b,
+                              double *out);
 hipError_t launch_force_sweep(const SweepShape &shape, const DevFields &f,
                               DevScalars *s, const PhysParams &prm,
                               hipStream_t stream);

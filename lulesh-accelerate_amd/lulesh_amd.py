@@ -536,9 +536,9 @@ class Domain:
 
 
 def device_math(op: str, a, b=None) -> np.ndarray:
-    """Evaluates the library's cube root, sqrt, division or the EOS's square
-    root (inputs >= 2^-767) on the GPU."""
-    code = {"cbrt": 0, "sqrt": 1, "div": 2, "sqrt_eos": 3}[op]
+    """Evaluates the library's cube root, sqrt, division, the EOS's square
+    root (inputs >= 2^-767) or the fused build's quotient on the GPU."""
+    code = {"cbrt": 0, "sqrt": 1, "div": 2, "sqrt_eos": 3, "div_fused": 4}[op]
     a = np.ascontiguousarray(a, np.float64)
     bb = np.ascontiguousarray(b, np.float64) if b is not None else None
     out = np.empty_like(a)

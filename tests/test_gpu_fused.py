@@ -113,3 +113,25 @@ def test_fused_mode_is_for_float64_cubes(lulesh_gpu, oracle):
     with pytest.raises(lulesh_gpu.LuleshError):
         u.set_fused_multiply_add(True)
     u.close()
+
+
+def test_fused_quotient(lulesh_gpu):
+    """The fused build divides by multiplying with the hardware reciprocal
+    refined by two Newton steps (elem_quot).  Against the correctly rounded
+    quotient, over both signs and operands from 2^-200 to 2^200 (the step's
+    divisors are volumes, masses, lengths and sound speeds of order 1e-10
+    to 1e10): within 2 ulp, and exact for the divisors that are powers of
+    two."""
+    rng = np.random.default_rng(5)
+    n = 400000
+    sign = lambda: rng.choice([-1.0, 1.0], n)
+    a = np.ldexp(1 + rng.uniform(0, 1, n), rng.integers(-200, 200, n)) * sign()
+    b = np.ldexp(1 + rng.uniform(0, 1, n), rng.integers(-200, 200, n)) * sign()
+    q = lulesh_gpu.device_math("div_fused", a, b)
+    exact = a / b
+    ulps = np.abs(q - exact) / np.spacing(np.abs(exact))
+    print(f"fused quotient: max {ulps.max():.2f} ulp, "
+          f"{np.mean(ulps == 0) * 100:.1f}% correctly rounded")
+    assert ulps.max() <= 2.0
+    p2 = np.ldexp(1.0, rng.integers(-200, 200, 1000))
+    assert np.array_equal(lulesh_gpu.device_math("div_fused", a[:1000], p2), a[:1000] / p2)
