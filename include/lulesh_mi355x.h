@@ -255,9 +255,10 @@ const char *lulesh_last_error(const lulesh_domain *d);
 
 /* The library's cube root (lulesh_cbrt.h) evaluated on the GPU (op 0), IEEE
  * sqrt (1) and division (2), the EOS's square root for inputs >= 2^-767
- * (3), and the quotient of the fused multiply-add build (4: the hardware
- * reciprocal refined by two Newton steps, times a), for n inputs: a
- * math-parity probe for the tests. */
+ * (3), and two quotients of the fused multiply-add build: a / b (4: the
+ * hardware reciprocal refined by two Newton steps, times a) and the
+ * hourglass coefficient's a / cbrt(b) for b > 0 normal (5: a times the
+ * reciprocal cube root), for n inputs: a math-parity probe for the tests. */
 int lulesh_device_math(int op, int n, const double *a, const double *b,
                        double *out);
 

@@ -1235,8 +1235,8 @@ const char *lulesh_last_error(const lulesh_domain *d)
 int lulesh_device_math(int op, int n, const double *a, const double *b,
                        double *out)
 {
-    const bool two = op == 2 || op == 4;
-    if (n < 0 || !a || !out || (two && !b) || op < 0 || op > 4) {
+    const bool two = op == 2 || op == 4 || op == 5;
+    if (n < 0 || !a || !out || (two && !b) || op < 0 || op > 5) {
         return LULESH_BAD_ARG;
     }
     if (n == 0) {
@@ -1255,10 +1255,10 @@ int lulesh_device_math(int op, int n, const double *a, const double *b,
         hipMemcpy(db, b, bytes, hipMemcpyHostToDevice) != hipSuccess)
         rc = LULESH_HIP_ERROR;
     if (rc == LULESH_OK) {
-        if (op == 4) {
-            // the fused build's quotient
+        if (op >= 4) {
+            // the fused build's quotient (4) and a / cbrt(b) (5)
             hipLaunchKernelGGL(fused::k_device_math, dim3((n + 255) / 256), dim3(256), 0, 0,
-                               2, n, da, db, dout);
+                               op == 4 ? 2 : 4, n, da, db, dout);
         } else {
             hipLaunchKernelGGL(k_device_math, dim3((n + 255) / 256), dim3(256), 0, 0,
                                op, n, da, db, dout);

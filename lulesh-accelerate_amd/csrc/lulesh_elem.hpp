@@ -71,6 +71,57 @@ __device__ __forceinline__ double elem_quot(double a, double b)
 }
 __device__ __forceinline__ float elem_quot(float a, float b) { return a / b; }
 
+// a / cbrt(d), d > 0 and normal, the hourglass coefficient, in two steps
+// with the root first (as the reference orders it): hg_root(d), then
+// hg_over_root(a, root).  The bit-identical build takes the project cube
+// root (lulesh_cbrt.h) and divides by it.  The fused build takes the
+// reciprocal cube root and multiplies by it: the project root's start
+// (d = t 2^(3q), t in [1, 8), the cubic fit of t^(-1/3)) and three of its
+// Newton steps z <- z + z (1 - t z^3) / 3 (relative error 2.6e-4 ->
+// 1.3e-7 -> 3.4e-14 -> rounding), then z 2^-q through the exponent field:
+// no root refinement and no divide.
+__device__ __forceinline__ double hg_root(double d)
+{
+#ifdef LULESH_FUSED
+    const uint64_t b = __double_as_longlong(d);
+    const int e = int((b >> 52) & 0x7ffu) - 1023;
+    const int q = int(unsigned(e + 1026) / 3u) - 342;  // floor(e / 3)
+    const int r = e - 3 * q;
+    const uint64_t frac = b & 0x000fffffffffffffull;
+    const double m = __longlong_as_double(frac | (1023ull << 52));
+    const double t = __longlong_as_double(frac | (uint64_t(1023 + r) << 52));
+    const double third = LULESH_CONST(0.3333333333333333);
+    double z = __builtin_fma(m, __builtin_fma(m, __builtin_fma(m, LULESH_CONST(-0.04993701447458462),
+                                                               LULESH_CONST(0.3177574893441185)),
+                                              LULESH_CONST(-0.809960645949834)),
+                             LULESH_CONST(1.5418870676993346));
+    const double zc = z * ((r == 1) ? LULESH_CONST(0.7937005259840998)
+                                    : LULESH_CONST(0.6299605249474366));
+    z = (r == 0) ? z : zc;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double dz = __builtin_fma(-t, (z * z) * z, 1.0);
+        z = __builtin_fma(z * dz, third, z);
+    }
+    return __longlong_as_double(__double_as_longlong(z) - (int64_t(q) << 52));
+#else
+    return lulesh_cbrt(d);
+#endif
+}
+__device__ __forceinline__ double hg_over_root(double a, double root)
+{
+#ifdef LULESH_FUSED
+    return a * root;
+#else
+    return a / root;
+#endif
+}
+__device__ __forceinline__ double elem_over_cbrt(double a, double d)
+{
+    const double root = hg_root(d);
+    return hg_over_root(a, root);
+}
+
 // fast square root of the arithmetic type: IEEE sqrt in double, the
 // hardware v_sqrt_f32 (1 ulp) in float
 __device__ __forceinline__ double elem_sqrt(double v) { return sqrt(v); }

@@ -378,9 +378,9 @@ __device__ __forceinline__ void sweep_tile(const Fields &f, DevScalars *s,
                 const double determ = field(3) * vv;
                 const double volinv = elem_recip(determ);
                 // the coefficient before hourgam, which holds 32 values live
-                const double volume13 = lulesh_cbrt(determ);
+                const double root = hg_root(determ);
                 const double coefficient =
-                    elem_quot(-prm.hgcoef * 0.01 * field(4) * field(5), volume13);
+                    hg_over_root(-prm.hgcoef * 0.01 * field(4) * field(5), root);
                 double hourgam[8][4];
                 {
                     double x[8], y[8], z[8];
@@ -645,8 +645,8 @@ __device__ __forceinline__ void hourglass_corners_u(const ElemU &E, const PhysPa
             }
         }
     }
-    const double volume13 = lulesh_cbrt(determ);
-    const double coefficient = elem_quot(-prm.hgcoef * 0.01 * E.ss * E.emass, volume13);
+    const double root = hg_root(determ);
+    const double coefficient = hg_over_root(-prm.hgcoef * 0.01 * E.ss * E.emass, root);
 #pragma unroll
     for (int t = 0; t < 3; ++t) {
         const double *u = E.vel[t];

@@ -1229,7 +1229,7 @@ __global__ void k_dt_finish_batch(const BatchMember *members, BatchWork *work,
 // device math probe: op 0 = lulesh_cbrt(a), 1 = sqrt(a), 2 = the element
 // arithmetic's quotient a / b (elem_quot: a divide here, the refined
 // reciprocal in the fused build), 3 = sqrt_above_tiny(a) (the EOS's square
-// root)
+// root), 4 = the hourglass coefficient's a / cbrt(b) (elem_over_cbrt)
 __global__ void k_device_math(int op, int n, const double *a, const double *b,
                               double *out)
 {
@@ -1243,6 +1243,8 @@ __global__ void k_device_math(int op, int n, const double *a, const double *b,
         out[t] = sqrt(a[t]);
     } else if (op == 3) {
         out[t] = sqrt_above_tiny(a[t]);
+    } else if (op == 4) {
+        out[t] = elem_over_cbrt(a[t], b[t]);
     } else {
         out[t] = elem_quot(a[t], b[t]);
     }

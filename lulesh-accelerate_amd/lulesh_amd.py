@@ -537,8 +537,10 @@ class Domain:
 
 def device_math(op: str, a, b=None) -> np.ndarray:
     """Evaluates the library's cube root, sqrt, division, the EOS's square
-    root (inputs >= 2^-767) or the fused build's quotient on the GPU."""
-    code = {"cbrt": 0, "sqrt": 1, "div": 2, "sqrt_eos": 3, "div_fused": 4}[op]
+    root (inputs >= 2^-767), or the fused build's a / b or a / cbrt(b)
+    (b > 0 normal) on the GPU."""
+    code = {"cbrt": 0, "sqrt": 1, "div": 2, "sqrt_eos": 3, "div_fused": 4,
+            "over_cbrt_fused": 5}[op]
     a = np.ascontiguousarray(a, np.float64)
     bb = np.ascontiguousarray(b, np.float64) if b is not None else None
     out = np.empty_like(a)

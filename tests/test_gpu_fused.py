@@ -135,3 +135,22 @@ def test_fused_quotient(lulesh_gpu):
     assert ulps.max() <= 2.0
     p2 = np.ldexp(1.0, rng.integers(-200, 200, 1000))
     assert np.array_equal(lulesh_gpu.device_math("div_fused", a[:1000], p2), a[:1000] / p2)
+
+
+def test_fused_over_cbrt(lulesh_gpu):
+    """The fused build's hourglass coefficient a / cbrt(d) is a times the
+    reciprocal cube root (elem_over_cbrt).  Against the quotient formed in
+    the host's extended precision (64-bit significand), over d > 0 from
+    2^-300 to 2^300 and both signs of a: within 3 ulp (2 measured: the
+    root's last Newton step and the product each round once)."""
+    assert np.finfo(np.longdouble).nmant >= 63
+    rng = np.random.default_rng(9)
+    n = 200000
+    a = np.ldexp(1 + rng.uniform(0, 1, n), rng.integers(-100, 100, n)) * rng.choice([-1.0, 1.0], n)
+    d = np.ldexp(1 + rng.uniform(0, 1, n), rng.integers(-300, 300, n))
+    d[:3] = [1.0, 8.0, 0.125]
+    got = lulesh_gpu.device_math("over_cbrt_fused", a, d)
+    ref = (a.astype(np.longdouble) / np.cbrt(d.astype(np.longdouble))).astype(np.float64)
+    ulps = np.abs(got - ref) / np.spacing(np.abs(ref))
+    print(f"fused a / cbrt(d): max {ulps.max():.2f} ulp, {np.mean(ulps == 0) * 100:.1f}% exact")
+    assert ulps.max() <= 3.0
