@@ -127,6 +127,14 @@ def cpu_baseline(nx: int, cycles: int) -> dict | None:
                       f"{cores} OpenMP threads"}
 
 
+# The mixed-precision form's algorithmic bytes in the same accounting: the
+# element arrays it keeps in float32 (p, q, ql, qq, ss, delv, arealg, vdov
+# and the six Q gradients) count 4 B, which takes 128 B off the element
+# side of every cycle (force 12, kinematics 12, Q gradients 24, Q region
+# 36, EOS 32, dt 12); the node side stays float64.
+MIXED_BYTES = (392, 296)
+
+
 def load_traffic(kernel: str, nx: int) -> float | None:
     """HBM bytes per launch of `kernel` measured by rocprofv3 --pmc (the
     L2-to-fabric read and write request counts by request size, an
@@ -140,6 +148,19 @@ def load_traffic(kernel: str, nx: int) -> float | None:
         return float(data[f"{kernel}@{nx}"]["bytes_per_launch"])
     except (KeyError, ValueError, TypeError):
         return None
+
+
+def cycle_traffic(nx: int, s_per_cycle: float) -> dict | None:
+    """The HBM bytes of a whole cycle as measured (the three phase kernels'
+    request-count traffic from profiles/traffic.json, fused build), per zone
+    and as a rate over the measured cycle time; None when not measured."""
+    per = [load_traffic(k, nx) for k in ("k_force_sweep", "k_kinematics_q", "k_q_eos_dt")]
+    if any(t is None for t in per):
+        return None
+    total = sum(per)
+    gbs = total / s_per_cycle / 1e9
+    return {"bytes_per_cycle": total, "bytes_per_zone_cycle": total / nx ** 3,
+            "achieved_gbs": gbs, "frac_of_peak": gbs / HBM_PEAK_GBS}
 
 
 def bench_batched(lulesh_amd, plain_value: float) -> dict:
@@ -336,10 +357,21 @@ def bench_mixed(lulesh_amd, nx: int = 120) -> dict:
     s_r, s_m = r.shock_radius(), m.shock_radius()
     for d in doms.values():
         d.close()
+    ne, nn = nx ** 3, (nx + 1) ** 3
+    b_mixed = MIXED_BYTES[0] * ne + MIXED_BYTES[1] * nn
+    b_fp64 = 520 * ne + 296 * nn
+    roof_mixed = HBM_PEAK_GBS * 1e9 / b_mixed * ne
+    roof_fp64 = HBM_PEAK_GBS * 1e9 / b_fp64 * ne
     return {
         "value": out["mixed"]["value"], "unit": "zones·cycles/s",
         "workload": f"sedov_{nx}^3_lulesh2.0_regions11 to t=1e-2",
         "fp64": out["fp64"], "mixed": out["mixed"], "fp64_fused": out["fp64_fused"],
+        # each against its own algorithmic bytes (float32 arrays at 4 B)
+        "roofline": {"b_alg_per_cycle": b_mixed, "roofline_zones_cycles_per_s": roof_mixed,
+                     "frac": out["mixed"]["value"] / roof_mixed,
+                     "fp64_b_alg_per_cycle": b_fp64,
+                     "fp64_frac": out["fp64"]["value"] / roof_fp64,
+                     "fp64_fused_frac": out["fp64_fused"]["value"] / roof_fp64},
         # against the bit-identical float64 step (the errors' reference) and
         # the float64 step with fused multiply-adds
         "vs_fp64_same_workload": out["mixed"]["value"] / out["fp64"]["value"],
@@ -384,6 +416,13 @@ def bench_learned_q(lulesh_amd, efactor: float = 1.0) -> dict:
                     "closure vs plain limiter, errors against the 120^3 run "
                     "restricted to each grid",
         "plain_40": plain, "learned_40": learned,
+        # the 40^3 domain's arrays (about 53 MB at 520 B per element and
+        # 296 B per node) stay in the GPU's 256 MB last-level cache, so an
+        # HBM fraction says nothing at this size; reported for completeness
+        "roofline": {"b_alg_per_cycle": 520 * 40 ** 3 + 296 * 41 ** 3,
+                     "frac": learned["zones_cycles_per_s"] / (HBM_PEAK_GBS * 1e9 / (
+                         520 * 40 ** 3 + 296 * 41 ** 3) * 40 ** 3),
+                     "note": "working set in last-level cache: not HBM-bound"},
         "fine_120_cycles": res["fine_cycles"],
         "plain_sweep": [{k: r[k] for k in ("nx",) + keep} for r in res["plain_sweep"]],
         # per metric: the plain resolution of equal error (interpolated
@@ -525,6 +564,8 @@ def main() -> int:
         "cycle_roofline": {"b_alg_per_cycle": b_cycle,
                            "roofline_zones_cycles_per_s": roof_value,
                            "frac": value / roof_value},
+        "cycle_traffic": cycle_traffic(nx, wall / K),
+        "grind_us_per_zone_cycle": 1e6 / value,
         "kernels_mean_ms": {k: round(v, 6) for k, v in per_kernel.items()},
         "gpu_event_ms_per_step": sum(ms for ms, _ in times.values()) / KT,
         "breakdown_cycles": fz["breakdown"],
