@@ -52,6 +52,9 @@ constexpr SweepShape kDefaultSweepMixed = {32, 8, 12, 3};
 constexpr int kBlock = 256;
 static_assert(kBlock == kEosBlockElems, "the Q/EOS block order is cut for this block size");
 constexpr int kBatch = 32;  // cycles queued per host round trip in run()
+// cycles per captured graph: lulesh_step replays a graph of this many cycles
+// as often as it fits and queues the rest one launch at a time
+constexpr int kGraphCycles = 32;
 
 // Node planes per block of the default sweep on an nx^3 domain: the largest
 // nz <= 17 whose grid still fills 90% of the resident block slots (two
@@ -129,6 +132,13 @@ struct lulesh_domain {
     long long kernel_launches[KID_COUNT] = {};
     hipEvent_t step_begin = nullptr, step_end = nullptr;
     double last_step_ms = 0.0;
+
+    // kGraphCycles cycles captured as one graph, and the launch state it was
+    // captured with (launch_signature): any change to what the cycle's
+    // launches depend on (the parameters, the mode, the network, the sweep
+    // shape) makes the next lulesh_step capture it again
+    hipGraphExec_t graph = nullptr;
+    std::vector<unsigned char> graph_sig;
 
     std::string err;
 };
@@ -295,6 +305,78 @@ int queue_cycle(lulesh_domain *d)
          })))
         return rc;
     return LULESH_OK;
+}
+
+// The bytes of everything queue_cycle reads from the domain: two equal
+// signatures queue the same launches with the same arguments.
+std::vector<unsigned char> launch_signature(const lulesh_domain *d)
+{
+    std::vector<unsigned char> sig;
+    auto put = [&sig](const void *p, size_t n) {
+        const unsigned char *b = static_cast<const unsigned char *>(p);
+        sig.insert(sig.end(), b, b + n);
+    };
+    const unsigned char flags[3] = {d->mixed, d->fused, d->unstructured};
+    put(flags, sizeof(flags));
+    put(&d->num_elem, sizeof(d->num_elem));
+    put(&d->eos_bands, sizeof(d->eos_bands));
+    put(&d->learned, sizeof(d->learned));
+    put(&d->learned_z_max, sizeof(d->learned_z_max));
+    put(&d->learned_form, sizeof(d->learned_form));
+    put(&d->f, sizeof(d->f));
+    put(&d->fm, sizeof(d->fm));
+    put(&d->s, sizeof(d->s));
+    put(&d->prm, sizeof(d->prm));
+    put(&d->sweep, sizeof(d->sweep));
+    put(&d->mesh, sizeof(d->mesh));
+    put(&d->fblocks, sizeof(d->fblocks));
+    return sig;
+}
+
+void drop_graph(lulesh_domain *d)
+{
+    if (d->graph) {
+        (void)hipGraphExecDestroy(d->graph);
+        d->graph = nullptr;
+    }
+    d->graph_sig.clear();
+}
+
+// the graph of kGraphCycles cycles for the current launch state, captured
+// from the domain's stream unless the one held matches it
+int ensure_graph(lulesh_domain *d)
+{
+    std::vector<unsigned char> sig = launch_signature(d);
+    if (d->graph && sig == d->graph_sig) {
+        return LULESH_OK;
+    }
+    drop_graph(d);
+    HIP_TRY(d, hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
+    int rc = LULESH_OK;
+    for (int c = 0; c < kGraphCycles && rc == LULESH_OK; ++c) {
+        rc = queue_cycle(d);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(d->stream, &g);
+    if (rc == LULESH_OK && e != hipSuccess) {
+        rc = fail(d, LULESH_HIP_ERROR,
+                  std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    }
+    if (rc == LULESH_OK) {
+        const hipError_t ei = hipGraphInstantiate(&d->graph, g, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            d->graph = nullptr;
+            rc = fail(d, LULESH_HIP_ERROR,
+                      std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        }
+    }
+    if (g) {
+        (void)hipGraphDestroy(g);
+    }
+    if (rc == LULESH_OK) {
+        d->graph_sig = std::move(sig);
+    }
+    return rc;
 }
 
 int read_scalars(lulesh_domain *d, DevScalars *out)
@@ -805,6 +887,7 @@ void lulesh_destroy(lulesh_domain *d)
     }
     if (d->step_begin) (void)hipEventDestroy(d->step_begin);
     if (d->step_end) (void)hipEventDestroy(d->step_end);
+    drop_graph(d);
     if (d->arena) (void)hipFree(d->arena);
     for (void *b : d->bufs) (void)hipFree(b);
     if (d->stream) (void)hipStreamDestroy(d->stream);
@@ -825,8 +908,20 @@ int lulesh_step(lulesh_domain *d, int ncycles, int *cycles_done)
     if (before.error != 0) {
         return fail(d, before.error, "domain is in an error state");
     }
+    // whole graphs of cycles first, then the rest (none while kernels are
+    // timed one by one: the timing brackets every launch with events).  The
+    // graph is brought up to date on every call, also on one too short to
+    // replay it, so that a caller's first calls capture it and later ones
+    // only replay it.
+    if (!d->timing && (rc = ensure_graph(d))) return rc;
     HIP_TRY(d, hipEventRecord(d->step_begin, d->stream));
-    for (int c = 0; c < ncycles; ++c) {
+    int c = 0;
+    if (!d->timing) {
+        for (; c + kGraphCycles <= ncycles; c += kGraphCycles) {
+            HIP_TRY(d, hipGraphLaunch(d->graph, d->stream));
+        }
+    }
+    for (; c < ncycles; ++c) {
         if ((rc = queue_cycle(d))) return rc;
     }
     // the last cycle's time constraints (the others were applied by the
