@@ -913,7 +913,7 @@ int lulesh_step(lulesh_domain *d, int ncycles, int *cycles_done)
     // graph is brought up to date on every call, also on one too short to
     // replay it, so that a caller's first calls capture it and later ones
     // only replay it.
-    if (!d->timing && (rc = ensure_graph(d))) return rc;
+    if (!d->timing && ncycles > 0 && (rc = ensure_graph(d))) return rc;
     HIP_TRY(d, hipEventRecord(d->step_begin, d->stream));
     int c = 0;
     if (!d->timing) {
