@@ -244,6 +244,14 @@ int lulesh_write_domain(lulesh_domain *d, const char *path);
 int lulesh_kernel_count(void);
 const char *lulesh_kernel_name(int k);
 int lulesh_set_kernel_timing(lulesh_domain *d, int enable);
+
+/* lulesh_step replays its cycles from a captured HIP graph of 32 cycles
+ * (captured again whenever a parameter, mode or launch shape changes), and
+ * queues the rest one launch at a time.  enable = 0 queues every cycle one
+ * launch at a time; the results are the same bits either way.  Default 1.
+ * No counterpart in the reference, whose Accelerate term is compiled and
+ * run by the Accelerate runtime. */
+int lulesh_set_cycle_graphs(lulesh_domain *d, int enable);
 int lulesh_kernel_times(lulesh_domain *d, double *ms, long long *launches);
 
 /* Milliseconds between the HIP events bracketing the last lulesh_step call

@@ -139,6 +139,7 @@ struct lulesh_domain {
     // shape) makes the next lulesh_step capture it again
     hipGraphExec_t graph = nullptr;
     std::vector<unsigned char> graph_sig;
+    bool graphs = true;  // lulesh_set_cycle_graphs
 
     std::string err;
 };
@@ -913,10 +914,11 @@ int lulesh_step(lulesh_domain *d, int ncycles, int *cycles_done)
     // graph is brought up to date on every call, also on one too short to
     // replay it, so that a caller's first calls capture it and later ones
     // only replay it.
-    if (!d->timing && ncycles > 0 && (rc = ensure_graph(d))) return rc;
+    const bool graphs = d->graphs && !d->timing;
+    if (graphs && ncycles > 0 && (rc = ensure_graph(d))) return rc;
     HIP_TRY(d, hipEventRecord(d->step_begin, d->stream));
     int c = 0;
-    if (!d->timing) {
+    if (graphs) {
         for (; c + kGraphCycles <= ncycles; c += kGraphCycles) {
             HIP_TRY(d, hipGraphLaunch(d->graph, d->stream));
         }
@@ -1206,6 +1208,18 @@ int lulesh_kernel_count(void) { return KID_COUNT; }
 const char *lulesh_kernel_name(int k)
 {
     return (k >= 0 && k < KID_COUNT) ? kKernelNames[k] : "";
+}
+
+int lulesh_set_cycle_graphs(lulesh_domain *d, int enable)
+{
+    if (!d) {
+        return LULESH_BAD_ARG;
+    }
+    d->graphs = enable != 0;
+    if (!d->graphs) {
+        drop_graph(d);
+    }
+    return LULESH_OK;
 }
 
 int lulesh_set_kernel_timing(lulesh_domain *d, int enable)

@@ -68,6 +68,7 @@ EXPORTS = (
     "lulesh_set_scalars", "lulesh_set_parameter", "lulesh_set_sweep_shape",
     "lulesh_get_sweep_shape", "lulesh_get_regions", "lulesh_symmetry",
     "lulesh_kernel_count", "lulesh_kernel_name", "lulesh_set_kernel_timing",
+    "lulesh_set_cycle_graphs",
     "lulesh_kernel_times", "lulesh_last_step_ms", "lulesh_last_error",
     "lulesh_device_math",
     "lulesh_ensemble_default", "lulesh_ensemble_draw", "lulesh_batch_create",
@@ -160,6 +161,8 @@ def lib() -> ctypes.CDLL:
     L.lulesh_kernel_name.restype = ctypes.c_char_p
     L.lulesh_set_kernel_timing.argtypes = [P, I]
     L.lulesh_set_kernel_timing.restype = I
+    L.lulesh_set_cycle_graphs.argtypes = [P, I]
+    L.lulesh_set_cycle_graphs.restype = I
     L.lulesh_kernel_times.argtypes = [P, DP, ctypes.POINTER(ctypes.c_longlong)]
     L.lulesh_kernel_times.restype = I
     L.lulesh_last_step_ms.argtypes = [P]
@@ -239,6 +242,14 @@ def default_config() -> Config:
     return c
 
 
+def _apply_cycle_graphs_env(d) -> None:
+    """LULESH_CYCLE_GRAPHS=0 in the environment turns the captured cycle
+    graphs off for every domain this module creates (for tools that trace
+    kernel launches one by one)."""
+    if os.environ.get("LULESH_CYCLE_GRAPHS", "1") == "0":
+        d.set_cycle_graphs(False)
+
+
 class Domain:
     """A Sedov cube of nx^3 elements resident on the GPU."""
 
@@ -271,6 +282,7 @@ class Domain:
         self.fused = False
         if fused:
             self.set_fused_multiply_add(True)
+        _apply_cycle_graphs_env(self)
         st = self.stats()
         self.num_elem, self.num_node = st.num_elem, st.num_node
 
@@ -321,6 +333,7 @@ class Domain:
         d = cls.__new__(cls)
         d.h = h
         d.nx, d.ic_mode, d.mixed, d.fused = 0, ic_mode, False, False
+        _apply_cycle_graphs_env(d)
         st = d.stats()
         d.num_elem, d.num_node = st.num_elem, st.num_node
         d.origin = lib().lulesh_origin_element(h)
@@ -520,6 +533,11 @@ class Domain:
                 "radius_x_axis": float(r[0, 0, ix])}
 
     # -- timing ---------------------------------------------------------
+    def set_cycle_graphs(self, enable: bool) -> None:
+        """Cycles replayed from captured graphs (default) or queued one
+        launch at a time (lulesh_set_cycle_graphs); the same bits."""
+        self._check(lib().lulesh_set_cycle_graphs(self.h, int(bool(enable))))
+
     def set_kernel_timing(self, enable: bool) -> None:
         self._check(lib().lulesh_set_kernel_timing(self.h, int(bool(enable))))
 

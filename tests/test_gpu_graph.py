@@ -1,8 +1,8 @@
 """Cycles replayed from a captured graph (lulesh_step replays graphs of 32
 cycles, then queues the rest) against the same cycles queued one launch at
-a time (kernel timing on, which bypasses the graphs): bit for bit, across
-remainders, a parameter change that must force a new capture, the other
-arithmetic modes and an unstructured mesh."""
+a time (kernel timing on, or lulesh_set_cycle_graphs off): bit for bit,
+across remainders, a parameter change that must force a new capture, the
+other arithmetic modes and an unstructured mesh."""
 import numpy as np
 import pytest
 
@@ -41,6 +41,22 @@ def test_graph_replay_is_direct_queueing(lulesh_gpu, kw):
     for d in (g, q):
         d.set_sweep_shape(32, 16, 4, 2)
     assert g.step(70) == q.step(70)
+    same(g, q)
+    g.close()
+    q.close()
+
+
+def test_graphs_off_is_the_same(lulesh_gpu):
+    """lulesh_set_cycle_graphs(d, 0) queues every cycle one launch at a
+    time, with the same bits; turned back on, the graph is captured anew."""
+    g = lulesh_gpu.Domain(20)
+    q = lulesh_gpu.Domain(20)
+    q.set_cycle_graphs(False)
+    for n in (64, 37):
+        assert g.step(n) == q.step(n)
+        same(g, q)
+    q.set_cycle_graphs(True)
+    assert g.step(96) == q.step(96)
     same(g, q)
     g.close()
     q.close()
