@@ -203,6 +203,23 @@ def bench_batched(lulesh_amd, plain_value: float) -> dict:
         d.close()
     print(f"bench: single-domain runs took {wall_plain:.2f} s", file=sys.stderr,
           flush=True)
+    # the same in the headline's arithmetic (fused multiply-adds and refined
+    # reciprocals), ensemble and single-domain path alike
+    b = lulesh_amd.Batch(nx, ef, fused=True)
+    t0 = time.perf_counter()
+    rc_f = b.run()
+    wall_f = time.perf_counter() - t0
+    zc_f = b.zone_cycles()
+    b.close()
+    wall_plain_f = 0.0
+    for n, f in zip(nx, ef):
+        d = lulesh_amd.Domain(int(n), efactor=float(f), fused=True)
+        t0 = time.perf_counter()
+        d.run()
+        wall_plain_f += time.perf_counter() - t0
+        d.close()
+    print(f"bench: fused ensemble ran in {wall_f:.2f} s, its single-domain runs "
+          f"took {wall_plain_f:.2f} s", file=sys.stderr, flush=True)
     value = zc / wall
     return {
         "value": value, "unit": "zones·cycles/s", "rc": rc,
@@ -220,6 +237,12 @@ def bench_batched(lulesh_amd, plain_value: float) -> dict:
         # as if the autotuner had run before the ensemble
         "vs_single_domain_path_with_autotune": wall_plain / (wall + t_tune),
         "vs_plain_150": value / plain_value,
+        "fused": {"value": zc_f / wall_f, "rc": rc_f, "wall_s": wall_f,
+                  "zone_cycles": zc_f,
+                  "same_sizes_single_domain_path": {"value": zc_f / wall_plain_f,
+                                                    "wall_s": wall_plain_f},
+                  "vs_single_domain_path": wall_plain_f / wall_f,
+                  "vs_plain_150": (zc_f / wall_f) / plain_value},
     }
 
 

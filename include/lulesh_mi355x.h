@@ -329,6 +329,12 @@ int lulesh_batch_get_shape(lulesh_batch *b, int *tx, int *ty, int *nz,
                            int *waves, int *sweep_blocks, int *elem_blocks,
                            int *cycles_per_graph);
 
+/* The phases with fused multiply-adds and refined reciprocals, as
+ * lulesh_set_fused_multiply_add for a single domain: every member is then
+ * bit for bit the fused single-domain run of its configuration.  Default 0
+ * (bit-identical to the reference). */
+int lulesh_batch_set_fused_multiply_add(lulesh_batch *b, int enable);
+
 /* The autotuner: times trial_cycles cycles of each candidate launch shape
  * (plain launches after one warm-up cycle) from a saved copy of the state,
  * restores the state and keeps the fastest shape with the batch's cycles

@@ -79,6 +79,7 @@ struct lulesh_batch {
 
     SweepShape sweep = kDefaultBatchSweep;
     int sweep_blocks = 0, elem_blocks = 0;
+    bool fused = false;  // the phases of lulesh::fused (lulesh_batch_set_fused_multiply_add)
     int cycles_per_graph = kDefaultCyclesPerGraph;
     int cu_count = 256;
 
@@ -217,30 +218,37 @@ int elem_grid(const lulesh_batch *b)
     return b->elem_blocks > 0 ? std::min(b->elem_blocks, b->full_chunks) : b->full_chunks;
 }
 
-// queues ncycles cycles and the closing kernel on the batch stream
+// queues ncycles cycles and the closing kernel on the batch stream; the
+// TimeIncrement and dt kernels are the same in both builds, as in the
+// single domain's cycle
 int queue_cycles(lulesh_batch *b, int ncycles)
 {
     hipStream_t st = b->stream;
     const int gs = sweep_grid(b), ge = elem_grid(b);
+    const bool fz = b->fused;
     for (int c = 0; c < ncycles; ++c) {
         hipLaunchKernelGGL(k_time_increment_batch, dim3(1), dim3(kTimeBlock), 0, st,
                            b->d_members, b->d_work, b->ic_mode);
         BHIP_TRY(b, hipGetLastError());
-        BHIP_TRY(b, launch_force_sweep_batch(b->sweep, gs, b->d_members, b->d_work,
-                                             b->prm, st));
+        BHIP_TRY(b, fz ? fused::launch_force_sweep_batch(b->sweep, gs, b->d_members,
+                                                         b->d_work, b->prm, st)
+                       : launch_force_sweep_batch(b->sweep, gs, b->d_members, b->d_work,
+                                                  b->prm, st));
         if (b->elem_blocks == 0) {
             // one chunk per block over the full chunk list
-            hipLaunchKernelGGL(k_kinematics_q_chunks, dim3(ge), dim3(kChunk), 0, st,
-                               b->d_members, b->d_work, b->prm);
+            hipLaunchKernelGGL(fz ? fused::k_kinematics_q_chunks : k_kinematics_q_chunks,
+                               dim3(ge), dim3(kChunk), 0, st, b->d_members, b->d_work,
+                               b->prm);
             BHIP_TRY(b, hipGetLastError());
-            hipLaunchKernelGGL(k_q_eos_dt_chunks, dim3(ge), dim3(kChunk), 0, st,
-                               b->d_members, b->d_work, b->prm);
+            hipLaunchKernelGGL(fz ? fused::k_q_eos_dt_chunks : k_q_eos_dt_chunks, dim3(ge),
+                               dim3(kChunk), 0, st, b->d_members, b->d_work, b->prm);
         } else {
-            hipLaunchKernelGGL(k_kinematics_q_batch, dim3(ge), dim3(kChunk), 0, st,
-                               b->d_members, b->d_work, b->prm);
+            hipLaunchKernelGGL(fz ? fused::k_kinematics_q_batch : k_kinematics_q_batch,
+                               dim3(ge), dim3(kChunk), 0, st, b->d_members, b->d_work,
+                               b->prm);
             BHIP_TRY(b, hipGetLastError());
-            hipLaunchKernelGGL(k_q_eos_dt_batch, dim3(ge), dim3(kChunk), 0, st,
-                               b->d_members, b->d_work, b->prm);
+            hipLaunchKernelGGL(fz ? fused::k_q_eos_dt_batch : k_q_eos_dt_batch, dim3(ge),
+                               dim3(kChunk), 0, st, b->d_members, b->d_work, b->prm);
         }
         BHIP_TRY(b, hipGetLastError());
     }
@@ -597,6 +605,18 @@ int lulesh_batch_get_field(lulesh_batch *b, int member, int field, double *host_
     BHIP_TRY(b, hipMemcpyAsync(host_out, p, size_t(count) * sizeof(double),
                                hipMemcpyDeviceToHost, b->stream));
     BHIP_TRY(b, hipStreamSynchronize(b->stream));
+    return LULESH_OK;
+}
+
+int lulesh_batch_set_fused_multiply_add(lulesh_batch *b, int enable)
+{
+    if (!b) {
+        return LULESH_BAD_ARG;
+    }
+    if (b->fused != (enable != 0)) {
+        b->fused = enable != 0;
+        drop_graph(b);
+    }
     return LULESH_OK;
 }
 

@@ -394,6 +394,17 @@ __global__ void k_device_math(int op, int n, const double *a, const double *b,
 hipError_t launch_force_sweep(const SweepShape &shape, const DevFields &f,
                               DevScalars *s, const PhysParams &prm,
                               hipStream_t stream);
+__global__ void k_kinematics_q_batch(const BatchMember *members,
+                                     const BatchWork *work, PhysParams prm);
+__global__ void k_q_eos_dt_batch(const BatchMember *members, const BatchWork *work,
+                                 PhysParams prm);
+__global__ void k_kinematics_q_chunks(const BatchMember *members, const BatchWork *work,
+                                      PhysParams prm);
+__global__ void k_q_eos_dt_chunks(const BatchMember *members, const BatchWork *work,
+                                  PhysParams prm);
+hipError_t launch_force_sweep_batch(const SweepShape &shape, int blocks,
+                                    const BatchMember *members, const BatchWork *work,
+                                    const PhysParams &prm, hipStream_t stream);
 }  // namespace fused
 
 }  // namespace lulesh

@@ -75,6 +75,7 @@ EXPORTS = (
     "lulesh_batch_destroy", "lulesh_batch_step", "lulesh_batch_run",
     "lulesh_batch_get_stats", "lulesh_batch_get_field",
     "lulesh_batch_set_shape", "lulesh_batch_get_shape",
+    "lulesh_batch_set_fused_multiply_add",
     "lulesh_batch_autotune", "lulesh_batch_last_ms",
     "lulesh_batch_last_error",
     "lulesh_mlp_param_count", "lulesh_mlp_create", "lulesh_mlp_destroy",
@@ -188,6 +189,8 @@ def lib() -> ctypes.CDLL:
     L.lulesh_batch_get_stats.restype = I
     L.lulesh_batch_get_field.argtypes = [P, I, I, DP]
     L.lulesh_batch_get_field.restype = I
+    L.lulesh_batch_set_fused_multiply_add.argtypes = [P, I]
+    L.lulesh_batch_set_fused_multiply_add.restype = I
     L.lulesh_batch_set_shape.argtypes = [P] + [I] * 7
     L.lulesh_batch_set_shape.restype = I
     L.lulesh_batch_get_shape.argtypes = [P] + [IP] * 7
@@ -597,7 +600,9 @@ class Batch:
 
     def __init__(self, nx, efactor=None, ic_mode: int = 2, num_reg: int = 11,
                  balance: int = 1, cost: int = 1, max_cycles: int = 9999999,
-                 stoptime: float = 1.0e-2):
+                 stoptime: float = 1.0e-2, fused: bool = False):
+        """fused=True runs the phases with fused multiply-adds
+        (lulesh_batch_set_fused_multiply_add)."""
         L = lib()
         nx = [int(v) for v in nx]
         n = len(nx)
@@ -620,6 +625,8 @@ class Batch:
         self.h = h
         self.nx = np.array(nx)
         self.n = n
+        if fused:
+            self._check(L.lulesh_batch_set_fused_multiply_add(h, 1))
 
     def close(self) -> None:
         if getattr(self, "h", None):

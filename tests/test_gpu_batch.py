@@ -132,3 +132,27 @@ def test_default_ensemble_members(lulesh_gpu, oracle):
     assert st.cycle == o.cycle
     for fld in ("e", "p", "xd", "x", "v", "ss"):
         assert np.array_equal(bits(b.get(m, fld)), bits(o.get(fld))), fld
+
+
+def test_fused_members_equal_fused_single_domains(lulesh_gpu):
+    """With fused multiply-adds (lulesh_batch_set_fused_multiply_add) every
+    member is bit for bit the fused single-domain run of its size and
+    energy factor, to t = 1e-2, and differs from the bit-identical one."""
+    nx = [9, 14, 11]
+    ef = [1.0, 0.87, 1.12]
+    b = lulesh_gpu.Batch(nx, ef, fused=True)
+    b.set_shape(16, 8, 4, 2, sweep_blocks=0, elem_blocks=0, cycles_per_graph=8)
+    assert b.run() == 0
+    e = lulesh_gpu.Batch(nx[:1], ef[:1])
+    assert e.run() == 0
+    assert not np.array_equal(bits(b.get(0, "e")), bits(e.get(0, "e")))
+    e.close()
+    for m, (n, f) in enumerate(zip(nx, ef)):
+        d = lulesh_gpu.Domain(n, efactor=f, fused=True)
+        d.run()
+        rc, st = b.stats(m)
+        assert rc == 0 and st.cycle == d.stats().cycle and st.time == d.stats().time
+        for fld in FIELDS:
+            assert np.array_equal(bits(b.get(m, fld)), bits(d.get(fld))), (n, f, fld)
+        d.close()
+    b.close()
