@@ -53,15 +53,12 @@ constexpr int kStencil[8][6] = {{1, 2, 3, 4, 5, 7}, {2, 3, 0, 5, 6, 4},
 // [by*(TY-1), (by+1)*(TY-1)) and node planes [bz*nz, (bz+1)*nz) of one
 // domain.
 //
-// Fast (the single domain's kernel): the element fields of the next layer
-// and the owned node's mass of the next plane are loaded one layer ahead
-// (the barriers keep the compiler from moving loads up, so a load issued in
-// the layer that uses it waits for HBM), and the owned node is copied to
-// registers so that the next node plane is stored inside the layer's last
-// fold: four barriers per layer.  Without it the kernel stores each plane at
-// the top of the layer (six barriers) and loads the fields where they are
-// used.  Both give the same bits.
-template <int TX, int TY, bool Fast, class Fields>
+// The element fields of the next layer and the owned node's mass of the
+// next plane are loaded one layer ahead (the barriers keep the compiler from
+// moving loads up, so a load issued in the layer that uses it waits for
+// HBM), and the owned node is copied to registers so that the next node
+// plane is stored inside the layer's last fold: four barriers per layer.
+template <int TX, int TY, class Fields>
 __device__ __forceinline__ void sweep_tile(const Fields &f, DevScalars *s,
                                            const PhysParams &prm, int nz, int bx,
                                            int by, int bz)
@@ -179,7 +176,7 @@ __device__ __forceinline__ void sweep_tile(const Fields &f, DevScalars *s,
     // slot takes plane k+2 after it.
     auto fold = [&](bool finalize, bool carry, bool layer, double *F, double *P, bool last,
                     auto &&advance) {
-        if (Fast && last) {
+        if (last) {
             advance(0);
         }
         __syncthreads();
@@ -211,7 +208,7 @@ __device__ __forceinline__ void sweep_tile(const Fields &f, DevScalars *s,
                 P[t] = acc;
             }
         }
-        if (Fast && last) {
+        if (last) {
             advance(1);
         }
         __syncthreads();
@@ -252,64 +249,36 @@ __device__ __forceinline__ void sweep_tile(const Fields &f, DevScalars *s,
     // (k < nx), k+1 (the other slot); both are in LDS when it starts, and
     // plane k+2 waits in registers
     const int kbeg = (kc0 > 0) ? kc0 - 1 : 0;
-    if constexpr (Fast) {
-        fetch_elem(kbeg);
-        fetch_mass(kbeg);
-    }
+    fetch_elem(kbeg);
+    fetch_mass(kbeg);
     fetch_plane(kbeg);
     store_plane(kbeg & 1);
     if (kbeg < nx) {
         fetch_plane(kbeg + 1);
-        if constexpr (Fast) {
-            store_plane((kbeg + 1) & 1);
-            if (kbeg + 1 < nx && kbeg + 1 < kc1) {
-                fetch_plane(kbeg + 2);
-            }
+        store_plane((kbeg + 1) & 1);
+        if (kbeg + 1 < nx && kbeg + 1 < kc1) {
+            fetch_plane(kbeg + 2);
         }
     }
-    if constexpr (Fast) {
-        __syncthreads();
-    }
+    __syncthreads();
     for (int k = kbeg; k < kc1; ++k) {
         const bool layer = k < nx;
-        if constexpr (!Fast) {
-            __syncthreads();  // the readers of the slot below are done
-            if (layer) {
-                store_plane((k + 1) & 1);  // plane k+1, fetched one layer ago
-            }
-            __syncthreads();
-            if (layer && k + 1 < nx && k + 1 < kc1) {
-                fetch_plane(k + 2);  // in flight while this layer computes
-            }
-        }
         const int lo = k & 1;
         const int hi = lo ^ 1;
         const bool finalize = owner && k >= kc0;
         const bool carry = owner && layer && k + 1 < kc1;
         const bool compute = layer && col_valid;
         // this layer's element fields (p q v volo ss elemMass) and the owned
-        // node's mass: prefetched (Fast), else loaded where they are used
+        // node's mass, prefetched one layer ago
         double ev[6];
-        double mass = 1.0;
-        if constexpr (Fast) {
 #pragma unroll
-            for (int i = 0; i < 6; ++i) {
-                ev[i] = epre[i];
-            }
-            mass = mpre;
-            fetch_elem(k + 1);
-            fetch_mass(k + 1);
+        for (int i = 0; i < 6; ++i) {
+            ev[i] = epre[i];
         }
-        const int el = compute ? (k * nx + je) * nx + ie : 0;
-        auto field = [&](int i) -> double {
-            if constexpr (Fast) {
-                return ev[i];
-            } else {
-                const typename Fields::CPtr a[6] = {f.p, f.q, f.v, f.volo, f.ss,
-                                                    f.elem_mass};
-                return a[i][el];
-            }
-        };
+        const double mass = mpre;
+        fetch_elem(k + 1);
+        fetch_mass(k + 1);
+        auto field = [&](int i) -> double { return ev[i]; };
         // the owned node of plane k (x y z xd yd zd), kept for the update
         double own[6];
         // before the last barrier of the layer: copy the owned node out of
@@ -455,13 +424,6 @@ __device__ __forceinline__ void sweep_tile(const Fields &f, DevScalars *s,
                 fz += F[5];
             }
             const int n = (k * en + je) * en + ie;
-            if constexpr (!Fast) {
-                mass = f.nodal_mass[n];
-#pragma unroll
-                for (int fld = 0; fld < 6; ++fld) {
-                    own[fld] = plane[lo][fld][base];
-                }
-            }
             double xdd = elem_quot(fx, mass);
             double ydd = elem_quot(fy, mass);
             double zdd = elem_quot(fz, mass);
@@ -488,7 +450,7 @@ template <int TX, int TY, int W>
 __global__ __launch_bounds__(TX *TY, W) void k_force_sweep(DevFields f, DevScalars *s,
                                                         PhysParams prm, int nz)
 {
-    sweep_tile<TX, TY, true>(f, s, prm, nz, int(blockIdx.x), int(blockIdx.y),
+    sweep_tile<TX, TY>(f, s, prm, nz, int(blockIdx.x), int(blockIdx.y),
                              int(blockIdx.z));
 }
 
@@ -504,7 +466,7 @@ __global__ __launch_bounds__(TX *TY, W) void k_force_sweep_batch(
     for (int t = int(blockIdx.x); t < n; t += int(gridDim.x)) {
         __syncthreads();  // the LDS of the previous tile is no longer read
         const BatchMember &m = members[tiles[4 * t]];
-        sweep_tile<TX, TY, true>(global_view(m.f), m.s, prm, nz, tiles[4 * t + 1],
+        sweep_tile<TX, TY>(global_view(m.f), m.s, prm, nz, tiles[4 * t + 1],
                                   tiles[4 * t + 2], tiles[4 * t + 3]);
     }
 }
