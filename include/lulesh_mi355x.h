@@ -272,8 +272,9 @@ int lulesh_device_math(int op, int n, const double *a, const double *b,
 
 /* ------------------------------------------------------------------------
  * Ensembles.  Many independent Sedov domains advanced together on one GPU:
- * each phase of a cycle is one persistent kernel over the tiles / element
- * chunks of all members, every member keeps its own time, dt and stop test
+ * each phase of a cycle is one kernel over the tiles / element chunks of
+ * all members (one block per item, or a persistent grid: the launch shape),
+ * every member keeps its own time, dt and stop test
  * (the loop of reference/C/lulesh-2.0.3/lulesh.cc:2771-2781 run per
  * member), and the cycle is replayed from a captured HIP graph.  The
  * reference runs one domain per process; a caller replaces N runs of

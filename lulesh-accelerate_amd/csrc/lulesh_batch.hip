@@ -4,9 +4,10 @@
 // All members live in one device arena.  A cycle is five launches whatever
 // the number of members: TimeIncrement of every member (one thread each),
 // the force sweep over the tile list of all members, kinematics and
-// Q / EOS / dt over the element-chunk list of all members, each as a
-// persistent grid, and one closing kernel that applies the last minima and
-// counts the members with cycles left.  Members run their own cycle loops:
+// Q / EOS / dt over the element-chunk lists of all members (one block per
+// tile or chunk, or a persistent grid of fewer blocks looping over the
+// list: a launch-shape choice), and one closing kernel that applies the last
+// minima and counts the members with cycles left.  Members run their own cycle loops:
 // a member whose time reached its stoptime (or that failed) takes no part;
 // the host drops its tiles and chunks from the lists when it notices.
 //

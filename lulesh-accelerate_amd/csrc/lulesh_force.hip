@@ -454,8 +454,9 @@ __global__ __launch_bounds__(TX *TY, W) void k_force_sweep(DevFields f, DevScala
                              int(blockIdx.z));
 }
 
-// The ensemble form: a persistent grid walks the tile list of all members;
-// a tile names (member, bx, by, bz).  Members whose run is over or failed
+// The ensemble form: a grid walks the tile list of all members (one block
+// per tile, or fewer blocks looping over it); a tile names (member, bx, by,
+// bz).  Members whose run is over or failed
 // leave their tiles at once (sweep_tile reads their flags).
 template <int TX, int TY, int W>
 __global__ __launch_bounds__(TX *TY, W) void k_force_sweep_batch(
