@@ -812,7 +812,10 @@ int lulesh_create_mesh(int num_elem, int num_node, const double *x, const double
                        const int *symm_z, int nfree, const int *free_surf,
                        const lulesh_config *cfg, lulesh_domain **out)
 {
-    if (num_elem < 1 || num_node < 8 || !x || !y || !z || !nodelist || !neighbors ||
+    // the bounds of the .lmesh reader: corner-major indices c * num_elem + el
+    // stay within int32 in the kernels
+    if (num_elem < 1 || num_node < 8 || num_elem > (1 << 26) || num_node > (1 << 27) ||
+        !x || !y || !z || !nodelist || !neighbors ||
         nsx < 0 || nsy < 0 || nsz < 0 || nfree < 0 || (nsx && !symm_x) ||
         (nsy && !symm_y) || (nsz && !symm_z) || (nfree && !free_surf)) {
         return LULESH_BAD_ARG;

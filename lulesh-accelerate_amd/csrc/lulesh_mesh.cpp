@@ -654,6 +654,9 @@ MeshLists read_lmesh(const char *path)
 HostMeshU build_mesh_u(const MeshLists &l)
 {
     const int ne = l.num_elem, nn = l.num_node;
+    if (ne > (1 << 26) || nn > (1 << 27)) {
+        throw std::invalid_argument("mesh: element / node counts out of range");
+    }
     if (ne < 1 || nn < 8 || l.x.size() != size_t(nn) || l.y.size() != size_t(nn) ||
         l.z.size() != size_t(nn) || l.nodelist.size() != size_t(ne) * 8 ||
         l.neighbors.size() != size_t(ne) * 6) {

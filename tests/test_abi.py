@@ -101,6 +101,17 @@ def test_host_side_rejections(lulesh_gpu, oracle, tmp_path):
     two = lulesh_gpu.default_config()
     assert lib.lulesh_create_lmesh(str(p).encode(), ctypes.byref(two), ctypes.byref(h)) == BAD
 
+    # the array form takes the file's count bounds (corner-major indices stay
+    # in int32); the counts are refused before any array is read
+    xs = np.zeros(8)
+    ids = np.zeros(8, dtype=np.int32)
+    xp = xs.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    ip = ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    for ne, nn in (((1 << 26) + 1, 8), (1, (1 << 27) + 1), (0, 8), (1, 7)):
+        rc = lib.lulesh_create_mesh(ne, nn, xp, xp, xp, ip, ip, 0, None, 0, None, 0, None,
+                                    0, None, ctypes.byref(one), ctypes.byref(h))
+        assert rc == BAD, (ne, nn)
+
     # the VTK writer's array form is for the n^3 cube
     z = np.zeros(8)
     dp = z.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
