@@ -302,16 +302,18 @@ def bench_unstructured(lulesh_amd, nx: int = 150, steps: int = 50, warmup: int =
     on-chip corner forces for their interior nodes) on the Sedov cube nx^3
     given as mesh lists, LULESH 1.0 ICs,
     beside the structured step on the same cube: zones*cycles/s over
-    `steps` cycles and the roofline of each unstructured kernel."""
+    `steps` cycles and the roofline of each unstructured kernel; then both
+    steps again with fused multiply-adds."""
     ne, nn = nx ** 3, (nx + 1) ** 3
     kt = 20
     out = {}
     lists = cube_lists(nx)
-    for name in ("unstructured", "structured"):
-        if name == "unstructured":
+    for name in ("unstructured", "structured", "unstructured_fused", "structured_fused"):
+        if name.startswith("unstructured"):
             d = lulesh_amd.Domain.from_mesh(lists, max_cycles=warmup + steps + kt)
         else:
             d = lulesh_amd.Domain(nx, ic_mode=1, max_cycles=warmup + steps + kt)
+        d.set_fused_multiply_add(name.endswith("_fused"))
         d.step(warmup)
         t0 = time.perf_counter()
         rc, n = d.step(steps)
@@ -348,6 +350,11 @@ def bench_unstructured(lulesh_amd, nx: int = 150, steps: int = 50, warmup: int =
                     f"{warmup + 1}-{warmup + steps}",
         "unstructured": u, "structured_same_cube": out["structured"],
         "vs_structured": u["value"] / out["structured"]["value"],
+        "fused": {"arithmetic": "float64 with fused multiply-adds, both steps",
+                  "unstructured": out["unstructured_fused"],
+                  "structured_same_cube": out["structured_fused"],
+                  "vs_structured": out["unstructured_fused"]["value"] /
+                  out["structured_fused"]["value"]},
         "cycle_roofline": {"b_alg_per_cycle": b_cycle, "roofline_zones_cycles_per_s": roof,
                            "frac": u["value"] / roof},
     }

@@ -159,12 +159,14 @@ int lulesh_run(lulesh_domain *d, int *cycles_done);
 /* Float64 arithmetic with fused multiply-adds (no counterpart in the
  * reference, whose C code rounds every product and every sum): enable = 1
  * lets the element phases (stress, hourglass, kinematics, Q gradients,
- * limiter, EOS) fuse a * b + c into one rounding, which makes the step about
- * 10% faster.  Results then differ from the reference by rounding only:
+ * limiter, EOS) fuse a * b + c into one rounding and take their quotients
+ * through refined hardware reciprocals, which makes the 150^3 step about 14%
+ * faster.  Results then differ from the reference by rounding only:
  * measured over full runs at 30^3 and 45^3, the same cycle counts, origin
- * energy within 1e-14 relative and fields within 2e-13 relative (the
+ * energy within 1e-14 relative and fields within 1e-11 relative (the
  * accepted tolerance is 1e-10).  enable = 0, the default, rounds every value
- * as the reference does, bit for bit.  Float64 structured domains only. */
+ * as the reference does, bit for bit.  Float64 domains, structured or
+ * unstructured; not the mixed-precision form. */
 int lulesh_set_fused_multiply_add(lulesh_domain *d, int enable);
 
 /* State in the Accelerate step's terms (src/Domain.hs:11-21): the domain

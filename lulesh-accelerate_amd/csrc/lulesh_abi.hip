@@ -221,27 +221,32 @@ int queue_cycle_u(lulesh_domain *d)
     const DevFields &f = d->f;
     const DevMeshU &m = d->mesh;
     const PhysParams &p = d->prm;
+    const bool fused = d->fused;
     int rc;
     if ((rc = launch(d, KID_TIME_INCREMENT, [&] {
              hipLaunchKernelGGL(k_time_increment, dim3(1), dim3(64), 0, st, s, p.ic_mode);
          })))
         return rc;
     if (b.num_blocks > 0 && (rc = launch(d, KID_FORCES_U, [&] {
-             hipLaunchKernelGGL(k_forces_u_blocks, dim3(unsigned(b.num_blocks)),
-                                dim3(kForceBlockElems), 0, st, f, m, b, s, p);
+             hipLaunchKernelGGL(fused ? fused::k_forces_u_blocks : k_forces_u_blocks,
+                                dim3(unsigned(b.num_blocks)), dim3(kForceBlockElems), 0, st,
+                                f, m, b, s, p);
          })))
         return rc;
     if (gb > 0 && (rc = launch(d, KID_NODES_U, [&] {
-                       hipLaunchKernelGGL(k_nodes_u, dim3(gb), dim3(kBlock), 0, st, f, m, s, p,
-                                          b.first_boundary, b.num_boundary);
+                       hipLaunchKernelGGL(fused ? fused::k_nodes_u : k_nodes_u, dim3(gb),
+                                          dim3(kBlock), 0, st, f, m, s, p, b.first_boundary,
+                                          b.num_boundary);
                    })))
         return rc;
     if ((rc = launch(d, KID_KINEMATICS_Q, [&] {
-             hipLaunchKernelGGL(k_kinematics_q_u, dim3(ge), dim3(kBlock), 0, st, f, m, s, p);
+             hipLaunchKernelGGL(fused ? fused::k_kinematics_q_u : k_kinematics_q_u, dim3(ge),
+                                dim3(kBlock), 0, st, f, m, s, p);
          })))
         return rc;
     if ((rc = launch(d, KID_Q_EOS_DT, [&] {
-             hipLaunchKernelGGL(k_q_eos_dt_u, dim3(ge), dim3(kBlock), 0, st, f, m, s, p);
+             hipLaunchKernelGGL(fused ? fused::k_q_eos_dt_u : k_q_eos_dt_u, dim3(ge),
+                                dim3(kBlock), 0, st, f, m, s, p);
          })))
         return rc;
     return LULESH_OK;
@@ -1271,9 +1276,9 @@ int lulesh_set_fused_multiply_add(lulesh_domain *d, int enable)
     if (!d) {
         return LULESH_BAD_ARG;
     }
-    if (d->mixed || d->unstructured) {
+    if (d->mixed) {
         return fail(d, LULESH_BAD_ARG,
-                    "fused multiply-adds are a mode of the float64 structured step");
+                    "fused multiply-adds are a mode of the float64 step");
     }
     d->fused = enable != 0;
     return LULESH_OK;

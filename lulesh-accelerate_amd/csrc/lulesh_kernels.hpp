@@ -405,6 +405,12 @@ __global__ void k_q_eos_dt_chunks(const BatchMember *members, const BatchWork *w
 hipError_t launch_force_sweep_batch(const SweepShape &shape, int blocks,
                                     const BatchMember *members, const BatchWork *work,
                                     const PhysParams &prm, hipStream_t stream);
+__global__ void k_forces_u_blocks(DevFields f, DevMeshU m, DevForceBlocksU b, DevScalars *s,
+                                  PhysParams prm);
+__global__ void k_nodes_u(DevFields f, DevMeshU m, DevScalars *s, PhysParams prm, int first,
+                          int count);
+__global__ void k_kinematics_q_u(DevFields f, DevMeshU m, DevScalars *s, PhysParams prm);
+__global__ void k_q_eos_dt_u(DevFields f, DevMeshU m, DevScalars *s, PhysParams prm);
 }  // namespace fused
 
 }  // namespace lulesh

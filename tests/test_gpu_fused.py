@@ -104,15 +104,64 @@ def test_fused_mode_is_in_effect_and_reproducible(lulesh_gpu):
         assert np.array_equal(f1[k], f2[k]), k
 
 
-def test_fused_mode_is_for_float64_cubes(lulesh_gpu, oracle):
+def test_fused_mode_is_for_float64(lulesh_gpu):
     d = lulesh_gpu.Domain(6, mixed=True)
     with pytest.raises(lulesh_gpu.LuleshError):
         d.set_fused_multiply_add(True)
     d.close()
-    u = lulesh_gpu.Domain.from_mesh(oracle.cube_mesh(4))
-    with pytest.raises(lulesh_gpu.LuleshError):
-        u.set_fused_multiply_add(True)
-    u.close()
+
+
+def test_fused_unstructured_sedov15oct(lulesh_gpu, oracle):
+    """The unstructured step with fused multiply-adds on the reference's
+    sedov15oct mesh, run to the end, against the oracle's run of the same
+    mesh: the reference's tolerances, and bits that differ from the exact
+    unstructured step's and repeat from run to run."""
+    path = os.path.join(GOLDEN, "sedov15oct.lmesh")
+    ref = oracle.Oracle.from_mesh(oracle.parse_lmesh(open(path).read()), cbrt_mode=0,
+                                  threads=8)
+    assert ref.run() == 0
+    runs = []
+    for fused in (False, True, True):
+        d = lulesh_gpu.Domain.from_lmesh(path)
+        d.set_fused_multiply_add(fused)
+        assert d.run()[0] == 0
+        if fused:
+            assert d.stats().cycle == ref.cycle
+            e0g, e0r = d.stats().e_origin, oracle.lib().lo_scalar(ref.h, b"e_origin")
+            assert abs(e0g - e0r) <= REL_TOL * abs(e0r), (e0g, e0r)
+            worst = {f: rel_norm(d.get(f), ref.get(f)) for f in NODE + ELEM}
+            print(f"fused sedov15oct: e(0) rel {abs(e0g - e0r) / abs(e0r):.2e}, worst field "
+                  f"{max(worst, key=worst.get)} {max(worst.values()):.2e}")
+            for f, r in worst.items():
+                assert r <= REL_TOL, (f, r)
+        runs.append({f: d.get(f).view(np.uint64) for f in NODE + ("e", "p", "q", "v")})
+        d.close()
+    exact, f1, f2 = runs
+    assert any(not np.array_equal(exact[k], f1[k]) for k in exact)
+    for k in f1:
+        assert np.array_equal(f1[k], f2[k]), k
+
+
+def test_fused_unstructured_cube(lulesh_gpu, oracle):
+    """The 24^3 cube given as lists (27 element blocks, most corner forces
+    summed on chip), fused, to the end of the run against the oracle's run
+    of the same lists."""
+    mesh = oracle.cube_mesh(24)
+    d = lulesh_gpu.Domain.from_mesh(mesh)
+    d.set_fused_multiply_add(True)
+    assert d.force_blocks()["blocks"] == 27
+    assert d.run()[0] == 0
+    ref = oracle.Oracle.from_mesh(mesh, cbrt_mode=0, threads=16)
+    assert ref.run() == 0
+    assert d.stats().cycle == ref.cycle
+    e0g, e0r = d.stats().e_origin, oracle.lib().lo_scalar(ref.h, b"e_origin")
+    assert abs(e0g - e0r) <= REL_TOL * abs(e0r), (e0g, e0r)
+    worst = {f: rel_norm(d.get(f), ref.get(f)) for f in NODE + ELEM}
+    print(f"fused 24^3 lists: cycles {ref.cycle}, e(0) rel {abs(e0g - e0r) / abs(e0r):.2e}, "
+          f"worst field {max(worst, key=worst.get)} {max(worst.values()):.2e}")
+    for f, r in worst.items():
+        assert r <= REL_TOL, (f, r)
+    d.close()
 
 
 def test_fused_quotient(lulesh_gpu):
