@@ -295,9 +295,9 @@ class Domain:
 
     @classmethod
     def from_lmesh(cls, path: str, stoptime: float = 1.0e-2,
-                   max_cycles: int = 9999999) -> "Domain":
+                   max_cycles: int = 9999999, fused: bool = False) -> "Domain":
         """An unstructured mesh read from a .lmesh file (LULESH 1.0
-        semantics)."""
+        semantics); fused as for the cube."""
         L = lib()
         cfg = default_config()
         cfg.ic_mode, cfg.stoptime, cfg.max_cycles = 1, stoptime, max_cycles
@@ -305,13 +305,14 @@ class Domain:
         rc = L.lulesh_create_lmesh(path.encode(), ctypes.byref(cfg), ctypes.byref(h))
         if rc != OK:
             raise LuleshError(rc, f"lulesh_create_lmesh failed for {path}")
-        return cls._wrap(h, 1)
+        return cls._wrap(h, 1, fused)
 
     @classmethod
     def from_mesh(cls, mesh: dict, stoptime: float = 1.0e-2,
-                  max_cycles: int = 9999999) -> "Domain":
+                  max_cycles: int = 9999999, fused: bool = False) -> "Domain":
         """An unstructured mesh from arrays: x, y, z, nodelist [ne, 8],
-        neighbors [ne, 6], symm_x, symm_y, symm_z, free_surf."""
+        neighbors [ne, 6], symm_x, symm_y, symm_z, free_surf; fused as for
+        the cube."""
         L = lib()
         cfg = default_config()
         cfg.ic_mode, cfg.stoptime, cfg.max_cycles = 1, stoptime, max_cycles
@@ -329,13 +330,15 @@ class Domain:
             ip(ints["free_surf"]), ctypes.byref(cfg), ctypes.byref(h))
         if rc != OK:
             raise LuleshError(rc, "lulesh_create_mesh failed")
-        return cls._wrap(h, 1)
+        return cls._wrap(h, 1, fused)
 
     @classmethod
-    def _wrap(cls, h, ic_mode: int) -> "Domain":
+    def _wrap(cls, h, ic_mode: int, fused: bool = False) -> "Domain":
         d = cls.__new__(cls)
         d.h = h
         d.nx, d.ic_mode, d.mixed, d.fused = 0, ic_mode, False, False
+        if fused:
+            d.set_fused_multiply_add(True)
         _apply_cycle_graphs_env(d)
         st = d.stats()
         d.num_elem, d.num_node = st.num_elem, st.num_node

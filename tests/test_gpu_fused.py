@@ -147,9 +147,8 @@ def test_fused_unstructured_cube(lulesh_gpu, oracle):
     summed on chip), fused, to the end of the run against the oracle's run
     of the same lists."""
     mesh = oracle.cube_mesh(24)
-    d = lulesh_gpu.Domain.from_mesh(mesh)
-    d.set_fused_multiply_add(True)
-    assert d.force_blocks()["blocks"] == 27
+    d = lulesh_gpu.Domain.from_mesh(mesh, fused=True)
+    assert d.fused and d.force_blocks()["blocks"] == 27
     assert d.run()[0] == 0
     ref = oracle.Oracle.from_mesh(mesh, cbrt_mode=0, threads=16)
     assert ref.run() == 0
