@@ -65,13 +65,19 @@ def test_fused_full_runs_within_reference_tolerance(lulesh_gpu, oracle, nx, mode
 
 
 def test_fused_benchmark_configuration(lulesh_gpu, oracle):
-    """150^3, 2.0 ICs, 11 regions: the 75 cycles the benchmark runs by
-    default (clock ramp, per-kernel timing, warm-up, timed cycles)."""
-    d = lulesh_gpu.Domain(150, ic_mode=2, max_cycles=75, fused=True)
-    assert d.run() == (0, 75)
+    """150^3, 2.0 ICs, 11 regions: the cycles the benchmark runs (clock
+    ramp, per-kernel timing, warm-up, timed cycles), checked after cycle 75
+    (the end of a --steps 20 --warmup 5 run) and 270 (the end of the
+    default --steps 200 --warmup 20 run, whose last 200 cycles are the
+    headline's)."""
+    d = lulesh_gpu.Domain(150, ic_mode=2, max_cycles=270, fused=True)
     ref = oracle.Oracle(150, mode=2, cbrt_mode=0, threads=16)
-    assert ref.run(75) == 0
-    check_against_reference(d, ref, oracle, 150, 2, symmetry=False)
+    done = 0
+    for upto in (75, 270):
+        assert d.step(upto - done) == (0, upto - done)
+        assert ref.run(upto) == 0
+        check_against_reference(d, ref, oracle, 150, 2, symmetry=False)
+        done = upto
     d.close()
 
 

@@ -210,17 +210,17 @@ def test_30_cubed_against_oracle(lulesh_gpu, oracle, mode):
 
 
 def test_150_cubed_benchmark_cycles_bitwise(lulesh_gpu, oracle):
-    """The benchmark configuration (150^3, 2.0 regions, 11 regions, balance
-    1, cost 1) bit for bit over every cycle the benchmark runs by default
-    with --steps 20 --warmup 5: 25 cycles to reach the GPU's sustained
-    clock, 25 of per-kernel timing, 5 warm-up and 20 timed cycles, checked
-    after cycle 3, 25 and 75.  max_cycles is the
-    reference's -i (the `cycle < its` loop test of
-    reference/C/lulesh-2.0.3/lulesh.cc:2771)."""
-    d, o = make_pair(lulesh_gpu, oracle, 2, 150, max_cycles=75)
+    """The benchmark configuration (150^3, 2.0 ICs, 11 regions, balance 1,
+    cost 1) bit for bit over every cycle the benchmark runs: 25 cycles to
+    reach the GPU's sustained clock, 25 of per-kernel timing, then W warm-up
+    and K timed cycles, checked after cycle 3, after 75 (the end of a
+    --steps 20 --warmup 5 run) and after 270 (the end of the default
+    --steps 200 --warmup 20 run).  max_cycles is the reference's -i (the
+    `cycle < its` loop test of reference/C/lulesh-2.0.3/lulesh.cc:2771)."""
+    d, o = make_pair(lulesh_gpu, oracle, 2, 150, max_cycles=270)
     np.testing.assert_array_equal(d.regions(), o.get_int("regNumList"))
     done = 0
-    for upto in (3, 25, 75):
+    for upto in (3, 75, 270):
         assert d.step(upto - done) == (0, upto - done)
         assert o.run(upto) == 0
         assert d.stats().cycle == o.cycle == upto
