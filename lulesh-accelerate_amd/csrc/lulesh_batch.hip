@@ -118,6 +118,9 @@ int bfail(lulesh_batch *b, int code, const std::string &msg)
 // the whole list is ordered as the single domain orders its blocks
 // (spread_costly_blocks): the costly chunks of all members spread over each
 // XCD's first slots, the rest member by member in mesh order.
+// The list has as many entries as the chunk list, whose size the device
+// copy of both is allocated for.
+static_assert(kChunk == kEosBlockElems, "a Q/EOS chunk is one Q/EOS block");
 void order_eos_chunks(lulesh_batch *b, bool all)
 {
     std::vector<int2> base;
