@@ -101,6 +101,16 @@ def test_host_side_rejections(lulesh_gpu, oracle, tmp_path):
     two = lulesh_gpu.default_config()
     assert lib.lulesh_create_lmesh(str(p).encode(), ctypes.byref(two), ctypes.byref(h)) == BAD
 
+    # ensembles: no members, mixed IC modes, no stop time, a negative cycle
+    # cap are refused before any device work
+    cfgs = (lulesh_gpu.Config * 2)(lulesh_gpu.default_config(), lulesh_gpu.default_config())
+    b = ctypes.c_void_p()
+    assert lib.lulesh_batch_create(cfgs, None, 0, ctypes.byref(b)) == BAD
+    for field, value in (("ic_mode", 1), ("stoptime", 0.0), ("max_cycles", -1)):
+        cfgs[1] = cfg_with(**{field: value})
+        assert lib.lulesh_batch_create(cfgs, None, 2, ctypes.byref(b)) == BAD, field
+    assert not b.value
+
     # the array form takes the file's count bounds (corner-major indices stay
     # in int32); the counts are refused before any array is read
     xs = np.zeros(8)
