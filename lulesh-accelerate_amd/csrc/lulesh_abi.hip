@@ -215,7 +215,8 @@ int queue_cycle_u(lulesh_domain *d)
 {
     const unsigned ge = unsigned((d->num_elem + kBlock - 1) / kBlock);
     const DevForceBlocksU &b = d->fblocks;  // a renumbered mesh
-    const unsigned gb = unsigned((b.num_boundary + kBlock - 1) / kBlock);
+    // k_nodes_u: eight lanes per boundary node (num_boundary <= 2^27)
+    const unsigned gb = unsigned((8 * int64_t(b.num_boundary) + kBlock - 1) / kBlock);
     hipStream_t st = d->stream;
     DevScalars *s = d->s;
     const DevFields &f = d->f;

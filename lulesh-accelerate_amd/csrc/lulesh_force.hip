@@ -792,69 +792,89 @@ __global__ __launch_bounds__(kForceBlockElems) void k_forces_u_blocks(DevFields 
     }
 }
 
-// The boundary nodes first .. first + count - 1, one per thread: gather of
-// their corners from their slots, acceleration, symmetry planes,
+// The boundary nodes first .. first + count - 1, eight lanes per node:
+// gather of their corners from their slots, acceleration, symmetry planes,
 // velocity and position (:905-926, :1475-1496, :1613-1683), in place (the
-// corner forces are complete before this kernel runs)
+// corner forces are complete before this kernel runs).  Lane q of a node's
+// group loads corners q, q + 8, ... of the node's run of slots, stress and
+// hourglass, so a wave's loads cover the consecutive runs of its eight nodes
+// sector after sector; the group's first lane adds them up from LDS in the
+// run's order and moves the node.
 __global__ __launch_bounds__(256) void k_nodes_u(DevFields f, DevMeshU m, DevScalars *s,
                                                  PhysParams prm, int first, int count)
 {
+    constexpr int kLanes = 8;
+    __shared__ double stage[6][256];  // one corner per lane: stress x y z, hourglass x y z
     if (s->active == 0 || s->error != 0) {
         return;
     }
-    const int i = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
+    const int t = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
+    const int i = t / kLanes, q = t % kLanes, l = int(threadIdx.x);
+    // a group lies in one wave and leaves whole: its lanes share i
     if (i >= count) {
         return;
     }
     const int n = first + i;
     const int par = s->cur;
     const double dt = s->deltatime;
+    const bool hourglass = prm.hgcoef > 0.0;
     // the node's corners are slots b .. e - 1, consecutive
     const int b = m.corner_start[n] - m.corner_base, e = m.corner_start[n + 1] - m.corner_base;
     const int cnt = e - b;
+    // every lane of the group loads the node (one line for the group), so
+    // that these loads are in flight with the corners'
     const NodeU nd = load_node_u(f, m, par, n);
-    // The first eight corners (all of them on a hexahedral mesh where at most
-    // eight elements meet) are loaded together, without a branch per corner:
-    // an absent one reads slot 0 and adds +0.0 instead, which leaves the
-    // sum's bits as skipping it does (a sum started from +0.0 is never
-    // -0.0).  Further corners follow in order.
-    auto gather = [&](const double *a, double &gx, double &gy, double &gz) {
-        double v[3][8];
+    // Each sum starts from 0.0 and adds the corners in slot order, as the
+    // reference adds a node's corners in element order.  A slot past the
+    // run adds +0.0, which leaves the sum's bits as skipping it does (a sum
+    // started from +0.0 is never -0.0).
+    double F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int c0 = 0; c0 < cnt; c0 += kLanes) {
+        double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (c0 + q < cnt) {
+            const size_t j = 4 * size_t(b + c0 + q);
+            const double2 *ps = reinterpret_cast<const double2 *>(m.bs + j);
+            const double2 sxy = ps[0], szw = ps[1];
+            v[0] = sxy.x;
+            v[1] = sxy.y;
+            v[2] = szw.x;
+            if (hourglass) {
+                const double2 *ph = reinterpret_cast<const double2 *>(m.bh + j);
+                const double2 hxy = ph[0], hzw = ph[1];
+                v[3] = hxy.x;
+                v[4] = hxy.y;
+                v[5] = hzw.x;
+            }
+        }
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const size_t j = size_t((c < cnt) ? b + c : 0);
-            const double2 *p = reinterpret_cast<const double2 *>(a + 4 * j);
-            const double2 xy = p[0], zw = p[1];
-            v[0][c] = xy.x;
-            v[1][c] = xy.y;
-            v[2][c] = zw.x;
+        for (int k = 0; k < 6; ++k) {
+            stage[k][l] = v[k];
         }
-        double ax = 0.0, ay = 0.0, az = 0.0;
+        // the group's lanes are in this wave, whose LDS accesses run in
+        // order: only the compiler must keep the reads after the writes
+        __builtin_amdgcn_wave_barrier();
+        if (q == 0) {
+            const int last = min(kLanes, cnt - c0);
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            ax += (c < cnt) ? v[0][c] : 0.0;
-            ay += (c < cnt) ? v[1][c] : 0.0;
-            az += (c < cnt) ? v[2][c] : 0.0;
+            for (int r = 0; r < kLanes; ++r) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    const double w = stage[k][l + r];
+                    F[k] += (r < last) ? w : 0.0;
+                }
+            }
         }
-        for (int j = b + 8; j < e; ++j) {
-            ax += a[4 * size_t(j)];
-            ay += a[4 * size_t(j) + 1];
-            az += a[4 * size_t(j) + 2];
-        }
-        gx = ax;
-        gy = ay;
-        gz = az;
-    };
-    double fx, fy, fz;
-    gather(m.bs, fx, fy, fz);
-    if (prm.hgcoef > 0.0) {
-        double hx, hy, hz;
-        gather(m.bh, hx, hy, hz);
-        fx += hx;
-        fy += hy;
-        fz += hz;
+        __builtin_amdgcn_wave_barrier();  // read before the next round writes
     }
-    move_node_u(f, par, dt, prm, n, nd, fx, fy, fz);
+    if (q == 0) {
+        double fx = F[0], fy = F[1], fz = F[2];
+        if (hourglass) {
+            fx += F[3];
+            fy += F[4];
+            fz += F[5];
+        }
+        move_node_u(f, par, dt, prm, n, nd, fx, fy, fz);
+    }
 }
 
 // (TX, TY, minimum waves per SIMD the register budget must allow); the
