@@ -393,29 +393,6 @@ __device__ __forceinline__ double mono_phi(double self, double delvm,
     return phi;
 }
 
-// minimum over the block of a value given as the bit pattern of a
-// non-negative double (ordered like the values), returned to thread 0
-__device__ __forceinline__ unsigned long long block_min(unsigned long long v,
-                                                        unsigned long long *lds)
-{
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        const unsigned long long o = __shfl_xor(v, off, 64);
-        v = (o < v) ? o : v;
-    }
-    const int wave = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        lds[wave] = v;
-    }
-    __syncthreads();
-    unsigned long long m = lds[0];
-    const int nw = (blockDim.x + 63) >> 6;
-    for (int w = 1; w < nw; ++w) {
-        m = (lds[w] < m) ? lds[w] : m;
-    }
-    return m;
-}
-
 }  // namespace
 
 // ---------------------------------------------------------------------------

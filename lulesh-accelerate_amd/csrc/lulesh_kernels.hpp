@@ -217,6 +217,61 @@ __device__ __forceinline__ double wave_uniform(double v)
     return __longlong_as_double((long long)((unsigned long long)hi << 32 | lo));
 }
 
+// One DPP step of a wavefront minimum: each lane keeps the smaller of its
+// value and that of the lane the DPP control picks in its row of 16.
+template <int Ctrl>
+__device__ __forceinline__ unsigned long long dpp_min_step(unsigned long long v)
+{
+    const unsigned lo =
+        unsigned(__builtin_amdgcn_update_dpp(0, int(unsigned(v)), Ctrl, 0xF, 0xF, false));
+    const unsigned hi =
+        unsigned(__builtin_amdgcn_update_dpp(0, int(unsigned(v >> 32)), Ctrl, 0xF, 0xF, false));
+    const unsigned long long o = (unsigned long long)hi << 32 | lo;
+    return o < v ? o : v;
+}
+
+// Minimum over the wavefront of 64-bit values (every lane active), exact
+// and order independent: DPP moves within each row of 16 lanes (swap
+// neighbours, swap pairs, mirror each half row, mirror the row), after
+// which every lane holds its row's minimum, then the four rows' minima read
+// from lanes 0, 16, 32 and 48.
+__device__ __forceinline__ unsigned long long wave_min(unsigned long long v)
+{
+    v = dpp_min_step<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+    v = dpp_min_step<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+    v = dpp_min_step<0x141>(v);  // row_half_mirror
+    v = dpp_min_step<0x140>(v);  // row_mirror
+    unsigned long long m = ~0ull;
+#pragma unroll
+    for (int r = 0; r < 64; r += 16) {
+        const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(unsigned(v)), r));
+        const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(unsigned(v >> 32)), r));
+        const unsigned long long w = (unsigned long long)hi << 32 | lo;
+        m = w < m ? w : m;
+    }
+    return m;
+}
+
+// Minimum over the block of a value given as the bit pattern of a
+// non-negative double (ordered like the values), returned to every thread;
+// every thread of the block calls it.
+__device__ __forceinline__ unsigned long long block_min(unsigned long long v,
+                                                        unsigned long long *lds)
+{
+    v = wave_min(v);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        lds[wave] = v;
+    }
+    __syncthreads();
+    unsigned long long m = lds[0];
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int w = 1; w < nw; ++w) {
+        m = (lds[w] < m) ? lds[w] : m;
+    }
+    return m;
+}
+
 // One of a node field's two parity copies.  The single domain's table is a
 // kernel argument and is indexed where it lies; the global view is a copy
 // in registers, which a run-time index would move to scratch memory, so its

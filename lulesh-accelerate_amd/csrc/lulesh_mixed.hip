@@ -451,27 +451,6 @@ __device__ __forceinline__ float mono_phi(float self, float delvm, float delvp,
     return phi;
 }
 
-__device__ __forceinline__ unsigned long long block_min(unsigned long long v,
-                                                        unsigned long long *lds)
-{
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        const unsigned long long o = __shfl_xor(v, off, 64);
-        v = (o < v) ? o : v;
-    }
-    const int wave = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        lds[wave] = v;
-    }
-    __syncthreads();
-    unsigned long long m = lds[0];
-    const int nw = (blockDim.x + 63) >> 6;
-    for (int w = 1; w < nw; ++w) {
-        m = (lds[w] < m) ? lds[w] : m;
-    }
-    return m;
-}
-
 // EOS of one element (reference/C/LULESH-OMP.cc:2384-2561; lulesh-2.0.3/
 // lulesh.cc:2043-2231): pressures, q and sound speed in float32, the energy
 // and its increments in float64
