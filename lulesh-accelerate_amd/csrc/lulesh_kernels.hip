@@ -397,14 +397,16 @@ __device__ __forceinline__ double mono_phi(double self, double delvm,
 
 // ---------------------------------------------------------------------------
 // Monotonic Q, EOS, volume update and time constraints of each element.
-// Threads walk the elements region by region (eos_order), so a wavefront
-// mostly shares one repetition count.  The neighbour across a face is the
-// adjacent element of the structured cube; x = 0, y = 0 and z = 0 faces are
-// symmetry planes (neighbour value = own value), the far faces free surfaces
-// (neighbour value = 0) (reference/C/LULESH-OMP.cc:3153-3171, :2204-2287).
-// The Courant and hydro minima are exact and order independent: each block
-// folds its minimum into the cycle's accumulator with an unsigned 64-bit
-// atomic min on the bit pattern, and the last block to finish applies them.
+// k_q_eos_dt_nat (the default) walks the elements in mesh order and shares
+// a block's EOS repetitions over its threads; k_q_eos_dt walks them region
+// by region (eos_order), so a wavefront mostly shares one repetition count.
+// The neighbour across a face is the adjacent element of the structured
+// cube; x = 0, y = 0 and z = 0 faces are symmetry planes (neighbour value =
+// own value), the far faces free surfaces (neighbour value = 0)
+// (reference/C/LULESH-OMP.cc:3153-3171, :2204-2287).  The Courant and hydro
+// minima are exact and order independent: each block folds its minimum into
+// the cycle's accumulator with an unsigned 64-bit atomic min on the bit
+// pattern, and the next kernel boundary applies them (apply_constraints).
 // ---------------------------------------------------------------------------
 namespace {
 
