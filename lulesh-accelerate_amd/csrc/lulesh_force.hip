@@ -680,6 +680,17 @@ __device__ __forceinline__ void store_corner(double *a, int slot, double x, doub
 
 }  // namespace
 
+// The bit-identical build writes a boundary corner's stress and hourglass
+// forces to two slot arrays, and k_nodes_u sums each kind in the node's
+// corner order, then adds the two sums, as the reference does.  The fused
+// build (results within the reference's tolerances, not its bits) writes
+// their sum to one slot array: half the boundary bytes, out and back.
+#ifdef LULESH_FUSED
+constexpr bool kOneSlot = true;
+#else
+constexpr bool kOneSlot = false;
+#endif
+
 // One block of elements (b.elem_start[blockIdx.x] ..., consecutive), one
 // element per thread, then the block's interior nodes (consecutive), one per
 // thread.  The stress corners and then the hourglass corners are staged in
@@ -735,6 +746,7 @@ __global__ __launch_bounds__(kForceBlockElems) void k_forces_u_blocks(DevFields 
         ce = b.ncorner_start[node + 1];
         nd = load_node_u(f, m, par, node);
     }
+    const bool hourglass = prm.hgcoef > 0.0;
     if (has) {
         double fs[3][8];
         if (stress_corners_u(E, fs)) {
@@ -746,7 +758,7 @@ __global__ __launch_bounds__(kForceBlockElems) void k_forces_u_blocks(DevFields 
             for (int t = 0; t < 3; ++t) {
                 corner[t][c][l] = fs[t][c];
             }
-            if (slot[c] >= 0) {
+            if (slot[c] >= 0 && (!kOneSlot || !hourglass)) {
                 store_corner(m.bs, slot[c], fs[0][c], fs[1][c], fs[2][c]);
             }
         }
@@ -759,18 +771,23 @@ __global__ __launch_bounds__(kForceBlockElems) void k_forces_u_blocks(DevFields 
         fy += corner[1][k & 7][k >> 3];
         fz += corner[2][k & 7][k >> 3];
     }
-    if (prm.hgcoef > 0.0) {
+    if (hourglass) {
         __syncthreads();  // the stress corners are summed
         if (has) {
             double fh[3][8];
             hourglass_corners_u(E, prm, fh);
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
+                if (kOneSlot && slot[c] >= 0) {
+                    // the element's own stress corner, still in LDS
+                    store_corner(m.bs, slot[c], corner[0][c][l] + fh[0][c],
+                                 corner[1][c][l] + fh[1][c], corner[2][c][l] + fh[2][c]);
+                }
 #pragma unroll
                 for (int t = 0; t < 3; ++t) {
                     corner[t][c][l] = fh[t][c];
                 }
-                if (slot[c] >= 0) {
+                if (!kOneSlot && slot[c] >= 0) {
                     store_corner(m.bh, slot[c], fh[0][c], fh[1][c], fh[2][c]);
                 }
             }
@@ -817,7 +834,8 @@ __global__ __launch_bounds__(256) void k_nodes_u(DevFields f, DevMeshU m, DevSca
     const int n = first + i;
     const int par = s->cur;
     const double dt = s->deltatime;
-    const bool hourglass = prm.hgcoef > 0.0;
+    // the fused build's slots hold stress plus hourglass
+    const bool hourglass = !kOneSlot && prm.hgcoef > 0.0;
     // the node's corners are slots b .. e - 1, consecutive
     const int b = m.corner_start[n] - m.corner_base, e = m.corner_start[n + 1] - m.corner_base;
     const int cnt = e - b;
